@@ -1,0 +1,11 @@
+"""ausm: MI355X-native explicit O-grid solver step (SLAU2 / SLAU / AUSM+-up,
+MUSCL, TVD-RK3) behind the C ABI of include/ausm.h."""
+from .lib import (AUSMUP, GRID_JACOBI, GRID_SOR, INIT_FREESTREAM, INIT_REST, MATH_EXACT,
+                  MATH_FAST, MINMOD, NOLIMITER, SLAU, SLAU2, VANALBADA, VANLEER, AusmError, load)
+from .solver import (Solver, circle_boundary, freestream, naca4_boundary, naca4_digits,
+                     rect_boundary)
+
+__all__ = ["Solver", "AusmError", "load", "naca4_boundary", "naca4_digits", "circle_boundary",
+           "rect_boundary", "freestream", "SLAU2", "SLAU", "AUSMUP", "VANALBADA", "MINMOD",
+           "VANLEER", "NOLIMITER", "MATH_EXACT", "MATH_FAST", "INIT_REST", "INIT_FREESTREAM",
+           "GRID_JACOBI", "GRID_SOR"]

@@ -1,0 +1,769 @@
+// C ABI of libausm: context management, geometry, grid generation, solver
+// parameters, stepping (with hipGraph capture of step chunks and a tile
+// autotuner) and read-back.  See include/ausm.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "aux_kernels.hpp"
+#include "ausm.h"
+#include "ctx.hpp"
+#include "kernels.hpp"
+
+using namespace ausm;
+
+namespace ausm {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return set_error(AUSM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Host-side parameter block -> device copy (the Uniforms upload,
+// main.js:735).  Derived fields for the fast path are formed in double.
+void fill_derived(DevParams& p) {
+  const double g = p.gamma;
+  p.gm1 = (float)(g - 1.0);
+  p.ggm1 = (float)(g / (g - 1.0));
+  p.minf_coef = (float)(((double)p.vx * p.vx + (double)p.vy * p.vy) / (g * (double)p.inP));
+  p.astar_coef = (float)(2.0 * (g - 1.0) / (g + 1.0));
+}
+
+void make_freestream(DevParams& p, double rho, double pr, double vx, double vy, double gamma) {
+  p.gamma = (float)gamma;
+  p.inRho = (float)rho;
+  p.inP = (float)pr;
+  p.vx = (float)vx;
+  p.vy = (float)vy;
+  const double rhoE = pr / (gamma - 1.0) + 0.5 * (vx * vx + vy * vy) * rho;
+  p.inState[0] = (float)rho;
+  p.inState[1] = (float)(vx * rho);
+  p.inState[2] = (float)(vy * rho);
+  p.inState[3] = (float)rhoE;
+  fill_derived(p);
+}
+
+// ---- tile autotuner -------------------------------------------------------
+// Rows per wavefront tile of the stage kernel, chosen per (M, N, scheme,
+// math) by timing one stage on the context's own scratch buffer (Q2, which
+// every step overwrites before reading), and remembered for the process.
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int, int>, int> g_tune;
+
+static const int kTileChoices[] = {1, 2, 4, 8};
+
+int autotune_rows(ausm_ctx* c) {
+  const auto key = std::make_tuple(c->M, c->N, c->scheme, c->math, c->limiter);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int best = 4;
+  float best_t = 1e30f;
+  for (int ty : kTileChoices) {
+    StageArgs a = stage_args(c, ty, 2);
+    a.qout = c->q[2];
+    for (int w = 0; w < 3; ++w) (void)launch_stage(a, c->scheme, c->math, ty, c->stream);
+    (void)hipEventRecord(e0, c->stream);
+    const int reps = 20;
+    for (int k = 0; k < reps; ++k) (void)launch_stage(a, c->scheme, c->math, ty, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_t) {
+      best_t = ms;
+      best = ty;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = best;
+  return best;
+}
+
+StageArgs stage_args(ausm_ctx* c, int ty, int stage) {
+  StageArgs a{};
+  a.vx = c->vx;
+  a.vy = c->vy;
+  a.par = c->dpar;
+  a.ws_in = c->ws_part;
+  a.ws_out = c->ws_part;
+  a.res_out = c->res_part;
+  a.maxws_cur = c->scal_f;
+  a.n_ws_in = c->n_ws_valid;
+  a.M = c->M;
+  a.N = c->N;
+  a.P = c->P;
+  a.PS = c->PS;
+  a.stage = stage;
+  a.limiter = c->limiter;
+  a.fresh = 0;
+  a.refresh = 0;
+  a.tiles_x = stage_tiles_x(c->M);
+  a.qin = c->q[0];
+  a.qn = c->q[0];
+  a.qout = c->q[1];
+  return a;
+}
+
+// Enqueue one TVD-RK3 step: three fused stage launches (the reference's 15
+// passes, main.js:744-777).
+hipError_t enqueue_step(ausm_ctx* c, int ty, bool fresh, bool refresh) {
+  StageArgs a = stage_args(c, ty, 1);
+  a.fresh = fresh;
+  a.qin = c->q[0];
+  a.qout = c->q[1];
+  hipError_t e = launch_stage(a, c->scheme, c->math, ty, c->stream);
+  if (e != hipSuccess) return e;
+  a = stage_args(c, ty, 2);
+  a.qin = c->q[1];
+  a.qn = c->q[0];
+  a.qout = c->q[2];
+  e = launch_stage(a, c->scheme, c->math, ty, c->stream);
+  if (e != hipSuccess) return e;
+  a = stage_args(c, ty, 3);
+  a.qin = c->q[2];
+  a.qn = c->q[0];
+  a.qout = c->q[0];
+  a.refresh = refresh;
+  return launch_stage(a, c->scheme, c->math, ty, c->stream);
+}
+
+void drop_graphs(ausm_ctx* c) {
+  for (auto& g : c->graphs) {
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+  }
+  c->graphs.clear();
+}
+
+}  // namespace ausm
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* ausm_last_error(void) { return g_err.c_str(); }
+int ausm_version(void) { return AUSM_VERSION; }
+
+#define CHECK_HIP(expr, what)                    \
+  do {                                           \
+    hipError_t _e = (expr);                      \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+int ausm_ctx_create(int device, int M, int N, ausm_ctx** out) {
+  if (!out) return set_error(AUSM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (M < 4 || N < 4 || M > (1 << 15) || N > (1 << 15))
+    return set_error(AUSM_ERR_ARG, "grid size %d x %d out of range (4..32768)", M, N);
+  int ndev = 0;
+  CHECK_HIP(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+  if (device < 0 || device >= ndev)
+    return set_error(AUSM_ERR_ARG, "device %d not present (%d devices)", device, ndev);
+  CHECK_HIP(hipSetDevice(device), "hipSetDevice");
+  ausm_ctx* c = new ausm_ctx();
+  c->device = device;
+  c->M = M;
+  c->N = N;
+  c->P = (int)round_up((size_t)M, 64);
+  c->PS = (int)round_up((size_t)N * c->P, 64);
+  int maxb = wave_speed_blocks(M, N);
+  for (int ty : kTileChoices) maxb = std::max(maxb, stage_blocks(M, N, ty));
+  c->max_parts = maxb;
+  const size_t nv = (size_t)(N + 1) * c->P;
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int k = 0; k < 3 && ok; ++k) ok = hipMalloc(&c->q[k], sizeof(float) * 4 * c->PS) == hipSuccess;
+  ok = ok && hipMalloc(&c->vx, sizeof(float) * nv) == hipSuccess;
+  ok = ok && hipMalloc(&c->vy, sizeof(float) * nv) == hipSuccess;
+  for (int k = 0; k < 4 && ok; ++k) ok = hipMalloc(&c->g64[k], sizeof(double) * nv) == hipSuccess;
+  ok = ok && hipMalloc(&c->ws_part, sizeof(float) * maxb) == hipSuccess;
+  ok = ok && hipMalloc(&c->res_part, sizeof(double) * maxb) == hipSuccess;
+  ok = ok && hipMalloc(&c->scal_f, sizeof(float) * 8) == hipSuccess;
+  ok = ok && hipMalloc(&c->scal_d, sizeof(double) * 8) == hipSuccess;
+  ok = ok && hipMalloc(&c->dpar, sizeof(DevParams)) == hipSuccess;
+  ok = ok && hipMalloc(&c->stage_buf, sizeof(float) * 4 * (size_t)M * (N + 3)) == hipSuccess;
+  ok = ok && hipMalloc(&c->seg, sizeof(double) * (2 * (size_t)M + M)) == hipSuccess;
+  ok = ok && hipHostMalloc(&c->pinned, 256) == hipSuccess;
+  if (!ok) {
+    ausm_ctx_destroy(c);
+    return set_error(AUSM_ERR_NOMEM, "device allocation failed for a %d x %d context", M, N);
+  }
+  // defaults: the reference's start-up uniforms (main.js:874-887, util.js:76-99)
+  make_freestream(c->hpar, 1.0, 1.0 / 1.4, 0.0, 0.0, 1.4);
+  c->hpar.maxdt = 1e-4f;
+  c->hpar.cfl = 1.5f;
+  c->hpar.Kp = 0.0f;
+  c->hpar.Ku = 0.75f;
+  c->scheme = SCHEME_SLAU2;
+  c->limiter = LIM_VANALBADA;
+  c->math = 0;
+  c->refresh = 1;
+  c->tile = 0;
+  c->use_graphs = getenv("AUSM_NO_GRAPHS") == nullptr;
+  (void)hipMemcpyAsync(c->dpar, &c->hpar, sizeof(DevParams), hipMemcpyHostToDevice, c->stream);
+  (void)hipMemsetAsync(c->scal_f, 0, sizeof(float) * 8, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    ausm_ctx_destroy(c);
+    return set_error(AUSM_ERR_HIP, "context initialisation failed");
+  }
+  *out = c;
+  return AUSM_OK;
+}
+
+int ausm_ctx_destroy(ausm_ctx* c) {
+  if (!c) return AUSM_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graphs(c);
+  for (int k = 0; k < 3; ++k) hipFree(c->q[k]);
+  hipFree(c->vx);
+  hipFree(c->vy);
+  for (int k = 0; k < 4; ++k) hipFree(c->g64[k]);
+  hipFree(c->ws_part);
+  hipFree(c->res_part);
+  hipFree(c->scal_f);
+  hipFree(c->scal_d);
+  hipFree(c->dpar);
+  hipFree(c->stage_buf);
+  hipFree(c->seg);
+  if (c->pinned) hipHostFree(c->pinned);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return AUSM_OK;
+}
+
+int ausm_ctx_stream(ausm_ctx* c, void** stream) {
+  if (!c || !stream) return set_error(AUSM_ERR_ARG, "NULL argument");
+  *stream = (void*)c->stream;
+  return AUSM_OK;
+}
+
+int ausm_sync(ausm_ctx* c) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return AUSM_OK;
+}
+
+// ---- geometry ---------------------------------------------------------------
+int ausm_boundary_naca4(double m, double p, double t, double size, double x_offset,
+                        double exponent, int spacing, int M, double* xy) {
+  if (!xy || M < 4) return set_error(AUSM_ERR_ARG, "bad NACA arguments");
+  if (m > 0 && (p <= 0 || p >= 1)) return set_error(AUSM_ERR_ARG, "camber position p must be in (0,1)");
+  const double chord = 2.0 * size;
+  for (int i = 0; i < M; ++i) {
+    // generateNACA4Boundary (grid.js:38-52): parameter s = i / (M/2)
+    const double s = (double)i / ((double)M / 2.0);
+    const bool upper = s <= 1.0;
+    double xc;
+    if (spacing == AUSM_SPACING_COSINE)
+      xc = upper ? 0.5 * (1.0 + std::cos(M_PI * s)) : 0.5 * (1.0 - std::cos(M_PI * (s - 1.0)));
+    else
+      xc = upper ? std::pow(1.0 - s, exponent) : std::pow(s - 1.0, exponent);
+    const double yt = 5.0 * t *
+                      (0.2969 * std::sqrt(xc) - 0.126 * xc - 0.3516 * xc * xc +
+                       0.2843 * xc * xc * xc - 0.1036 * xc * xc * xc * xc);
+    double yc = 0.0, dyc = 0.0;
+    if (m > 0) {
+      if (xc < p) {
+        yc = (m / (p * p)) * (2.0 * p * xc - xc * xc);
+        dyc = (m / (p * p)) * (2.0 * p - 2.0 * xc);
+      } else {
+        yc = (m / ((1.0 - p) * (1.0 - p))) * ((1.0 - 2.0 * p) + 2.0 * p * xc - xc * xc);
+        dyc = (m / ((1.0 - p) * (1.0 - p))) * (2.0 * p - 2.0 * xc);
+      }
+    }
+    // thickness applied perpendicular to the mean line
+    const double th = std::atan(dyc);
+    const double sx = yt * -std::sin(th), sy = yt / std::sqrt(1.0 + dyc * dyc);
+    const double sgn = upper ? 1.0 : -1.0;
+    xy[2 * i] = (xc - 0.5 + sgn * sx) * chord + x_offset;
+    xy[2 * i + 1] = (yc + sgn * sy) * chord;
+  }
+  return AUSM_OK;
+}
+
+int ausm_boundary_circle(double radius, int M, double* xy) {
+  if (!xy || M < 4 || !(radius > 0)) return set_error(AUSM_ERR_ARG, "bad circle arguments");
+  for (int i = 0; i < M; ++i) {
+    const double t = (double)i / (double)M;  // grid.js:232-237
+    xy[2 * i] = radius * std::cos(t * M_PI * 2.0);
+    xy[2 * i + 1] = radius * std::sin(t * M_PI * 2.0);
+  }
+  return AUSM_OK;
+}
+
+int ausm_boundary_rect(double ratio, int M, double* xy) {
+  if (!xy || M < 4 || !(ratio > 0)) return set_error(AUSM_ERR_ARG, "bad rectangle arguments");
+  for (int i = 0; i < M; ++i) {
+    double t = std::fmod((double)i / (double)M + 0.125, 1.0);  // grid.js:238-255
+    double x, y;
+    if (t < 0.25) {
+      t *= 4.0;
+      x = ratio;
+      y = t * 2.0 - 1.0;
+    } else if (t < 0.5) {
+      t = (t - 0.25) * 4.0;
+      x = ((1.0 - t) * 2.0 - 1.0) * ratio;
+      y = 1.0;
+    } else if (t < 0.75) {
+      t = (t - 0.5) * 4.0;
+      x = -ratio;
+      y = (1.0 - t) * 2.0 - 1.0;
+    } else {
+      t = (t - 0.75) * 4.0;
+      x = (t * 2.0 - 1.0) * ratio;
+      y = -1.0;
+    }
+    xy[2 * i] = x;
+    xy[2 * i + 1] = y;
+  }
+  return AUSM_OK;
+}
+
+// ---- grid -------------------------------------------------------------------
+static int finish_grid(ausm_ctx* c, const double* gx, const double* gy) {
+  const size_t nv = (size_t)(c->N + 1) * c->P;
+  CHECK_HIP(launch_f64_to_f32(gx, c->vx, nv, c->stream), "grid convert");
+  CHECK_HIP(launch_f64_to_f32(gy, c->vy, nv, c->stream), "grid convert");
+  c->has_grid = true;
+  c->has_state = false;
+  return AUSM_OK;
+}
+
+int ausm_grid_from_boundary(ausm_ctx* c, const double* body, const double* outer, int iters,
+                            int method, double omega) {
+  if (!c || !body || !outer || iters < 0) return set_error(AUSM_ERR_ARG, "bad grid arguments");
+  if (method != AUSM_GRID_JACOBI && method != AUSM_GRID_SOR)
+    return set_error(AUSM_ERR_ARG, "unknown grid method %d", method);
+  if (method == AUSM_GRID_SOR && (c->M % 2 != 0))
+    return set_error(AUSM_ERR_ARG, "red-black SOR needs an even number of points around (M=%d)", c->M);
+  hipSetDevice(c->device);
+  const int M = c->M, N = c->N, P = c->P;
+  // boundary rows into FP64 planes (writeTexture, main.js:665-670)
+  std::vector<double> hx((size_t)(N + 1) * P, 0.0), hy((size_t)(N + 1) * P, 0.0);
+  for (int i = 0; i < M; ++i) {
+    hx[i] = body[2 * i];
+    hy[i] = body[2 * i + 1];
+    hx[(size_t)N * P + i] = outer[2 * i];
+    hy[(size_t)N * P + i] = outer[2 * i + 1];
+  }
+  const size_t bytes = sizeof(double) * hx.size();
+  double *gx = c->g64[0], *gy = c->g64[1], *hx2 = c->g64[2], *hy2 = c->g64[3];
+  CHECK_HIP(hipMemcpyAsync(gx, hx.data(), bytes, hipMemcpyHostToDevice, c->stream), "grid upload");
+  CHECK_HIP(hipMemcpyAsync(gy, hy.data(), bytes, hipMemcpyHostToDevice, c->stream), "grid upload");
+  CHECK_HIP(launch_grid_interp(gx, gy, M, N, P, c->stream), "grid interpolation");
+  if (method == AUSM_GRID_JACOBI) {
+    for (int it = 0; it < iters; ++it) {
+      CHECK_HIP(launch_grid_jacobi(gx, gy, hx2, hy2, M, N, P, c->stream), "grid Jacobi");
+      std::swap(gx, hx2);
+      std::swap(gy, hy2);
+    }
+  } else {
+    for (int it = 0; it < iters; ++it) {
+      CHECK_HIP(launch_grid_sor(gx, gy, M, N, P, 0, omega, c->stream), "grid SOR");
+      CHECK_HIP(launch_grid_sor(gx, gy, M, N, P, 1, omega, c->stream), "grid SOR");
+    }
+  }
+  c->g_cur = (gx == c->g64[0]) ? 0 : 2;
+  const int r = finish_grid(c, gx, gy);
+  if (r != AUSM_OK) return r;
+  CHECK_HIP(hipStreamSynchronize(c->stream), "grid generation");
+  return AUSM_OK;
+}
+
+int ausm_grid_set_vertices(ausm_ctx* c, const float* xy) {
+  if (!c || !xy) return set_error(AUSM_ERR_ARG, "NULL argument");
+  hipSetDevice(c->device);
+  const int M = c->M, N = c->N, P = c->P;
+  std::vector<float> hx((size_t)(N + 1) * P, 0.0f), hy((size_t)(N + 1) * P, 0.0f);
+  for (int j = 0; j <= N; ++j)
+    for (int i = 0; i < M; ++i) {
+      hx[(size_t)j * P + i] = xy[((size_t)j * M + i) * 2];
+      hy[(size_t)j * P + i] = xy[((size_t)j * M + i) * 2 + 1];
+    }
+  CHECK_HIP(hipMemcpyAsync(c->vx, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
+  CHECK_HIP(hipMemcpyAsync(c->vy, hy.data(), sizeof(float) * hy.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "vertex upload");
+  c->has_grid = true;
+  c->has_state = false;
+  c->g_cur = -1;
+  return AUSM_OK;
+}
+
+int ausm_grid_get_vertices(ausm_ctx* c, float* xy) {
+  if (!c || !xy) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "no grid");
+  hipSetDevice(c->device);
+  const int M = c->M, N = c->N, P = c->P;
+  std::vector<float> hx((size_t)(N + 1) * P), hy((size_t)(N + 1) * P);
+  CHECK_HIP(hipMemcpyAsync(hx.data(), c->vx, sizeof(float) * hx.size(), hipMemcpyDeviceToHost, c->stream), "vertex download");
+  CHECK_HIP(hipMemcpyAsync(hy.data(), c->vy, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, c->stream), "vertex download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "vertex download");
+  for (int j = 0; j <= N; ++j)
+    for (int i = 0; i < M; ++i) {
+      xy[((size_t)j * M + i) * 2] = hx[(size_t)j * P + i];
+      xy[((size_t)j * M + i) * 2 + 1] = hy[(size_t)j * P + i];
+    }
+  return AUSM_OK;
+}
+
+int ausm_grid_get_vertices64(ausm_ctx* c, double* xy) {
+  if (!c || !xy) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (c->g_cur < 0) return set_error(AUSM_ERR_ORDER, "no FP64 grid (vertices were uploaded)");
+  hipSetDevice(c->device);
+  const int M = c->M, N = c->N, P = c->P;
+  std::vector<double> hx((size_t)(N + 1) * P), hy((size_t)(N + 1) * P);
+  CHECK_HIP(hipMemcpyAsync(hx.data(), c->g64[c->g_cur], sizeof(double) * hx.size(), hipMemcpyDeviceToHost, c->stream), "grid download");
+  CHECK_HIP(hipMemcpyAsync(hy.data(), c->g64[c->g_cur + 1], sizeof(double) * hy.size(), hipMemcpyDeviceToHost, c->stream), "grid download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "grid download");
+  for (int j = 0; j <= N; ++j)
+    for (int i = 0; i < M; ++i) {
+      xy[((size_t)j * M + i) * 2] = hx[(size_t)j * P + i];
+      xy[((size_t)j * M + i) * 2 + 1] = hy[(size_t)j * P + i];
+    }
+  return AUSM_OK;
+}
+
+int ausm_grid_get_metrics(ausm_ctx* c, float* area, float* flen) {
+  if (!c || !area || !flen) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "no grid");
+  hipSetDevice(c->device);
+  const int M = c->M, N = c->N;
+  float* da = c->stage_buf;  // M*N
+  float* df = c->q[2];       // M*N*4: the scratch stage buffer Q2 is dead between steps
+  CHECK_HIP(launch_metrics(c->vx, c->vy, M, N, c->P, da, df, c->stream), "metrics");
+  CHECK_HIP(hipMemcpyAsync(area, da, sizeof(float) * M * N, hipMemcpyDeviceToHost, c->stream), "metrics download");
+  CHECK_HIP(hipMemcpyAsync(flen, df, sizeof(float) * 4 * M * N, hipMemcpyDeviceToHost, c->stream), "metrics download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "metrics");
+  return AUSM_OK;
+}
+
+// ---- parameters ---------------------------------------------------------------
+static int upload_params(ausm_ctx* c) {
+  CHECK_HIP(hipMemcpyAsync(c->dpar, &c->hpar, sizeof(DevParams), hipMemcpyHostToDevice, c->stream), "parameter upload");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "parameter upload");
+  return AUSM_OK;
+}
+
+int ausm_set_freestream(ausm_ctx* c, double rho, double p, double vx, double vy, double gamma) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (!(rho > 0) || !(p > 0) || !(gamma > 1) || !std::isfinite(vx) || !std::isfinite(vy))
+    return set_error(AUSM_ERR_ARG, "bad free stream (rho=%g p=%g gamma=%g)", rho, p, gamma);
+  hipSetDevice(c->device);
+  make_freestream(c->hpar, rho, p, vx, vy, gamma);
+  return upload_params(c);
+}
+
+int ausm_set_scheme(ausm_ctx* c, int scheme, int limiter, float K_p, float K_u) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (scheme < 0 || scheme > 2) return set_error(AUSM_ERR_ARG, "unknown scheme %d", scheme);
+  if (limiter < 0 || limiter > 3) return set_error(AUSM_ERR_ARG, "unknown limiter %d", limiter);
+  hipSetDevice(c->device);
+  if (scheme != c->scheme || limiter != c->limiter) drop_graphs(c);
+  c->scheme = scheme;
+  c->limiter = limiter;
+  c->hpar.Kp = K_p;
+  c->hpar.Ku = K_u;
+  return upload_params(c);
+}
+
+int ausm_set_cfl(ausm_ctx* c, float cfl, float maxdt, int refresh) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (!(cfl > 0) || !(maxdt >= 0) || refresh < 1)
+    return set_error(AUSM_ERR_ARG, "bad CFL settings (cfl=%g maxdt=%g refresh=%d)", cfl, maxdt, refresh);
+  hipSetDevice(c->device);
+  if (refresh != c->refresh) drop_graphs(c);
+  c->hpar.cfl = cfl;
+  c->hpar.maxdt = maxdt;
+  c->refresh = refresh;
+  return upload_params(c);
+}
+
+int ausm_set_math(ausm_ctx* c, int math) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (math != AUSM_MATH_EXACT && math != AUSM_MATH_FAST)
+    return set_error(AUSM_ERR_ARG, "unknown math mode %d", math);
+  if (math != c->math) drop_graphs(c);
+  c->math = math;
+  return AUSM_OK;
+}
+
+int ausm_set_tile(ausm_ctx* c, int rows) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8)
+    return set_error(AUSM_ERR_ARG, "rows per wavefront must be 0 (auto), 1, 2, 4 or 8");
+  if (rows != c->tile) drop_graphs(c);
+  c->tile = rows;
+  return AUSM_OK;
+}
+
+// ---- state --------------------------------------------------------------------
+static int refresh_ws(ausm_ctx* c) {
+  // wave speeds of the current solution -> per-block maxima (the post pass
+  // of prepareState, main.js:649-652)
+  CHECK_HIP(launch_wave_speed(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, nullptr,
+                              c->ws_part, c->stream),
+            "wave speeds");
+  c->n_ws_valid = wave_speed_blocks(c->M, c->N);
+  c->ws_fresh = true;
+  return AUSM_OK;
+}
+
+int ausm_init_state(ausm_ctx* c, int mode) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "set a grid before the state");
+  if (mode != AUSM_INIT_REST && mode != AUSM_INIT_FREESTREAM)
+    return set_error(AUSM_ERR_ARG, "unknown init mode %d", mode);
+  hipSetDevice(c->device);
+  CHECK_HIP(launch_fill_state(c->q[0], c->dpar, c->M, c->N, c->P, c->PS, mode, c->stream), "init state");
+  int r = refresh_ws(c);
+  if (r != AUSM_OK) return r;
+  CHECK_HIP(hipStreamSynchronize(c->stream), "init state");
+  c->has_state = true;
+  c->steps = 0;
+  return AUSM_OK;
+}
+
+int ausm_set_state(ausm_ctx* c, const float* q) {
+  if (!c || !q) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "set a grid before the state");
+  hipSetDevice(c->device);
+  const size_t n = 4 * (size_t)c->M * (c->N + 3);
+  CHECK_HIP(hipMemcpyAsync(c->stage_buf, q, sizeof(float) * n, hipMemcpyHostToDevice, c->stream), "state upload");
+  CHECK_HIP(launch_aos_to_soa(c->stage_buf, c->q[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+  int r = refresh_ws(c);
+  if (r != AUSM_OK) return r;
+  CHECK_HIP(hipStreamSynchronize(c->stream), "state upload");
+  c->has_state = true;
+  c->steps = 0;
+  return AUSM_OK;
+}
+
+int ausm_get_state(ausm_ctx* c, float* q) {
+  if (!c || !q) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  const size_t n = 4 * (size_t)c->M * (c->N + 3);
+  CHECK_HIP(launch_soa_to_aos(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, c->stage_buf, c->stream), "state download");
+  CHECK_HIP(hipMemcpyAsync(q, c->stage_buf, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "state download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "state download");
+  return AUSM_OK;
+}
+
+static int current_rows(ausm_ctx* c) { return c->tile ? c->tile : autotune_rows(c); }
+
+int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (nsteps < 0) return set_error(AUSM_ERR_ARG, "negative step count");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "initialise the state before stepping");
+  hipSetDevice(c->device);
+  const int ty = current_rows(c);
+  const int K = c->refresh;
+  // graph chunk: a multiple of the dt-refresh cadence, about 32 steps
+  const int G = K * std::max(1, 32 / K);
+  int left = nsteps;
+  while (left > 0) {
+    const bool aligned = (c->steps % K) == 0 && c->ws_fresh;
+    if (aligned && left >= G && c->use_graphs) {
+      GraphEntry* ge = nullptr;
+      for (auto& g : c->graphs)
+        if (g.ty == ty && g.steps == G && g.n_ws_in == c->n_ws_valid) ge = &g;
+      if (!ge) {
+        // capture G steps; within the chunk step s is fresh iff s % K == 0
+        GraphEntry g{};
+        g.ty = ty;
+        g.steps = G;
+        g.n_ws_in = c->n_ws_valid;
+        CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        const int save_valid = c->n_ws_valid;
+        hipError_t e = hipSuccess;
+        for (int s = 0; s < G && e == hipSuccess; ++s) {
+          const bool fresh = (s % K) == 0;
+          const bool refresh = ((s + 1) % K) == 0;
+          e = enqueue_step(c, ty, fresh, refresh);
+          if (refresh) c->n_ws_valid = stage_blocks(c->M, c->N, ty);
+        }
+        hipGraph_t graph;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+        c->n_ws_valid = save_valid;
+        if (e != hipSuccess) return hip_fail(e, "graph capture (launch)");
+        if (e2 != hipSuccess) return hip_fail(e2, "graph capture");
+        g.graph = graph;
+        CHECK_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        c->graphs.push_back(g);
+        ge = &c->graphs.back();
+      }
+      CHECK_HIP(hipGraphLaunch(ge->exec, c->stream), "graph launch");
+      c->steps += G;
+      left -= G;
+      c->n_ws_valid = stage_blocks(c->M, c->N, ty);
+      c->ws_fresh = true;
+    } else {
+      const bool fresh = c->ws_fresh;
+      const bool refresh = ((c->steps + 1) % K) == 0;
+      CHECK_HIP(enqueue_step(c, ty, fresh, refresh), "stage launch");
+      c->ws_fresh = refresh;
+      if (refresh) c->n_ws_valid = stage_blocks(c->M, c->N, ty);
+      c->steps += 1;
+      left -= 1;
+    }
+  }
+  if (info) {
+    // density-residual norm of the last step's stage 1 and the current max
+    // wave speed; one small read-back
+    CHECK_HIP(launch_reduce_sum_f64(c->res_part, stage_blocks(c->M, c->N, ty), c->scal_d, c->stream), "residual norm");
+    CHECK_HIP(launch_reduce_max(c->ws_part, c->n_ws_valid, c->scal_f + 1, c->stream), "max wave speed");
+    CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "info");
+    CHECK_HIP(hipMemcpyAsync((char*)c->pinned + 8, c->scal_f, 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "info");
+    CHECK_HIP(hipStreamSynchronize(c->stream), "step");
+    const double l2 = *(double*)c->pinned;
+    const float cur = ((float*)((char*)c->pinned + 8))[0];
+    const float nxt = ((float*)((char*)c->pinned + 8))[1];
+    info->res_l2 = nsteps > 0 ? l2 : -1.0;
+    info->max_wave_speed = cur;
+    info->dt = std::fmin(c->hpar.maxdt, c->hpar.cfl / cur);
+    info->nonfinite = !(std::isfinite(cur) && std::isfinite(nxt)) || !std::isfinite(l2);
+    info->steps_done = c->steps;
+    if (info->nonfinite) return set_error(AUSM_ERR_NONFINITE, "non-finite solution after %lld steps", c->steps);
+  } else {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "step");
+  }
+  return AUSM_OK;
+}
+
+int ausm_eval_residual(ausm_ctx* c, float* res) {
+  if (!c || !res) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  const int ty = current_rows(c);
+  StageArgs a = stage_args(c, ty, 0);
+  a.qin = c->q[0];
+  a.qout = c->q[2];  // scratch: every step rewrites Q2 before reading it
+  CHECK_HIP(launch_stage(a, c->scheme, c->math, ty, c->stream), "residual");
+  CHECK_HIP(launch_planes_to_aos(c->q[2], c->M, c->N, c->P, c->PS, c->stage_buf, c->stream), "residual");
+  CHECK_HIP(hipMemcpyAsync(res, c->stage_buf, sizeof(float) * 4 * (size_t)c->M * c->N, hipMemcpyDeviceToHost, c->stream), "residual download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "residual");
+  return AUSM_OK;
+}
+
+int ausm_wave_speeds(ausm_ctx* c, float* ws, float* max_ws) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  float* dws = c->stage_buf;
+  float* part = c->stage_buf + (size_t)c->M * c->N;
+  const int nb = wave_speed_blocks(c->M, c->N);
+  CHECK_HIP(launch_wave_speed(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, dws, part, c->stream), "wave speeds");
+  CHECK_HIP(launch_reduce_max(part, nb, c->scal_f + 2, c->stream), "max wave speed");
+  if (ws) CHECK_HIP(hipMemcpyAsync(ws, dws, sizeof(float) * (size_t)c->M * c->N, hipMemcpyDeviceToHost, c->stream), "wave speed download");
+  CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_f + 2, sizeof(float), hipMemcpyDeviceToHost, c->stream), "wave speed download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "wave speeds");
+  if (max_ws) *max_ws = *(float*)c->pinned;
+  return AUSM_OK;
+}
+
+int ausm_forces(ausm_ctx* c, double ref_length, double* out, double* cp) {
+  if (!c || !out) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!(ref_length > 0)) return set_error(AUSM_ERR_ARG, "reference length must be positive");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  double* dcp = c->seg + 2 * (size_t)c->M;
+  CHECK_HIP(launch_forces(c->q[0], c->vx, c->vy, c->dpar, c->M, c->PS, ref_length, c->seg, c->scal_d + 2, dcp, c->stream), "forces");
+  CHECK_HIP(hipMemcpyAsync(out, c->scal_d + 2, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "forces download");
+  if (cp) CHECK_HIP(hipMemcpyAsync(cp, dcp, sizeof(double) * c->M, hipMemcpyDeviceToHost, c->stream), "cp download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "forces");
+  return AUSM_OK;
+}
+
+int ausm_residual_norm(ausm_ctx* c, double* l2) {
+  if (!c || !l2) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  // a stage-1 launch into the scratch buffer Q1 accumulates the per-block
+  // density-residual sums of the current solution
+  const int ty = current_rows(c);
+  StageArgs a = stage_args(c, ty, 1);
+  a.qin = c->q[0];
+  a.qout = c->q[1];
+  a.fresh = 0;
+  CHECK_HIP(launch_stage(a, c->scheme, c->math, ty, c->stream), "residual norm");
+  CHECK_HIP(launch_reduce_sum_f64(c->res_part, stage_blocks(c->M, c->N, ty), c->scal_d, c->stream), "residual norm");
+  CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "residual norm");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "residual norm");
+  *l2 = *(double*)c->pinned;
+  return AUSM_OK;
+}
+
+int ausm_profile_stages(ausm_ctx* c, int reps, float* ms3) {
+  if (!c || !ms3 || reps < 1) return set_error(AUSM_ERR_ARG, "bad profiling arguments");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  const int ty = current_rows(c);
+  hipEvent_t e0, e1;
+  CHECK_HIP(hipEventCreate(&e0), "event");
+  CHECK_HIP(hipEventCreate(&e1), "event");
+  for (int st = 1; st <= 3; ++st) {
+    StageArgs a = stage_args(c, ty, st);
+    a.qin = st == 1 ? c->q[0] : (st == 2 ? c->q[1] : c->q[2]);
+    a.qn = c->q[0];
+    a.qout = st == 2 ? c->q[2] : c->q[1];  // scratch outputs only
+    a.fresh = 0;
+    a.refresh = st == 3;
+    a.ws_out = c->ws_part + 0;  // refresh writes partials; restored below
+    double total = 0.0;
+    for (int k = 0; k < reps; ++k) {
+      CHECK_HIP(hipEventRecord(e0, c->stream), "event");
+      CHECK_HIP(launch_stage(a, c->scheme, c->math, ty, c->stream), "stage");
+      CHECK_HIP(hipEventRecord(e1, c->stream), "event");
+      CHECK_HIP(hipEventSynchronize(e1), "event");
+      float ms = 0.0f;
+      CHECK_HIP(hipEventElapsedTime(&ms, e0, e1), "event");
+      total += ms;
+    }
+    ms3[st - 1] = (float)(total / reps);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the stage-3 launches above overwrote the wave-speed partials: recompute
+  // them from the (unchanged) solution so that the next step is unaffected
+  if (c->ws_fresh) {
+    const int r = refresh_ws(c);
+    if (r != AUSM_OK) return r;
+  }
+  CHECK_HIP(hipStreamSynchronize(c->stream), "profile");
+  return AUSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,386 @@
+// Setup, epilogue and layout kernels of libausm:
+//   * FP64 elliptic grid generation: linear interpolation
+//     (src/shaders/grid.js:15-28), Jacobi sweeps of the Winslow / TTM
+//     equations with P = Q = 0 (grid.js:44-110) and a red-black SOR variant;
+//   * wave speeds + per-workgroup maxima for the CFL time step
+//     (src/shaders/ausm.js:617-630, 535-582);
+//   * surface forces summed in FP64 (src/shaders/graphs.js:16-101);
+//   * FP64 reduction of the per-workgroup residual sums;
+//   * conversions between the reference's AoS texture layout and the SoA
+//     planes the stage kernel streams, with ghost rows formed as the
+//     boundary pass (ausm.js:34-120) forms them.
+#include "aux_kernels.hpp"
+
+namespace ausm {
+
+// --------------------------------------------------------------------------
+// grid
+// --------------------------------------------------------------------------
+__global__ void grid_interp_kernel(double* gx, double* gy, int M, int N, int P) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y + 1;
+  if (x >= M || y >= N) return;
+  const double t = (double)y / (double)N;
+  const double ox = gx[x], oy = gy[x];
+  const double bx = gx[N * P + x], by = gy[N * P + x];
+  gx[y * P + x] = ox * (1.0 - t) + bx * t;
+  gy[y * P + x] = oy * (1.0 - t) + by * t;
+}
+
+// One Winslow/TTM point update with P = Q = 0 (shaders/grid.js:79-108).
+// Returns false when the local Jacobian is not negative (point kept).
+__device__ __forceinline__ bool ttm_point(const double* gx, const double* gy, int M, int P, int x,
+                                          int y, double& nx, double& ny) {
+  const int l = (x == 0) ? M - 1 : x - 1;
+  const int r = (x + 1 == M) ? 0 : x + 1;
+  const int u = (y + 1) * P, c = y * P, d = (y - 1) * P;
+  const double X[9] = {gx[c + x], gx[c + l], gx[c + r], gx[d + x], gx[u + x],
+                       gx[d + l], gx[d + r], gx[u + l], gx[u + r]};
+  const double Y[9] = {gy[c + x], gy[c + l], gy[c + r], gy[d + x], gy[u + x],
+                       gy[d + l], gy[d + r], gy[u + l], gy[u + r]};
+  // 0 center, 1 left, 2 right, 3 down, 4 up, 5 downleft, 6 downright, 7 upleft, 8 upright
+  const double xix = 0.5 * (X[2] - X[1]), xiy = 0.5 * (Y[2] - Y[1]);
+  const double etx = 0.5 * (X[4] - X[3]), ety = 0.5 * (Y[4] - Y[3]);
+  const double xex = 0.25 * (((X[8] + X[5]) - X[7]) - X[6]);
+  const double xey = 0.25 * (((Y[8] + Y[5]) - Y[7]) - Y[6]);
+  const double alpha = etx * etx + ety * ety;
+  const double gam = xix * xix + xiy * xiy;
+  const double beta = xix * etx + xiy * ety;
+  const double jac = xix * ety - xiy * etx;
+  const double den = 2.0 * (alpha + gam);
+  const double npx =
+      (alpha * ((X[1] + X[2]) + 0.0 * xix) + gam * ((X[3] + X[4]) + 0.0 * etx) - (2.0 * beta) * xex) /
+      den;
+  const double npy =
+      (alpha * ((Y[1] + Y[2]) + 0.0 * xiy) + gam * ((Y[3] + Y[4]) + 0.0 * ety) - (2.0 * beta) * xey) /
+      den;
+  if (jac < 0.0) {
+    // mix(center, newPos, omega = 1)
+    nx = X[0] * (1.0 - 1.0) + npx * 1.0;
+    ny = Y[0] * (1.0 - 1.0) + npy * 1.0;
+    return true;
+  }
+  nx = X[0];
+  ny = Y[0];
+  return false;
+}
+
+__global__ void grid_jacobi_kernel(const double* gx, const double* gy, double* ox, double* oy, int M,
+                                   int N, int P) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y > N) return;
+  if (y == 0 || y == N) {
+    ox[y * P + x] = gx[y * P + x];
+    oy[y * P + x] = gy[y * P + x];
+    return;
+  }
+  double nx, ny;
+  ttm_point(gx, gy, M, P, x, y, nx, ny);
+  ox[y * P + x] = nx;
+  oy[y * P + x] = ny;
+}
+
+// Red-black successive over-relaxation of the same update (in place).
+__global__ void grid_sor_kernel(double* gx, double* gy, int M, int N, int P, int color,
+                                double omega) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y + 1;
+  if (x >= M || y >= N || ((x + y) & 1) != color) return;
+  double nx, ny;
+  if (ttm_point(gx, gy, M, P, x, y, nx, ny)) {
+    const double cx = gx[y * P + x], cy = gy[y * P + x];
+    gx[y * P + x] = cx + omega * (nx - cx);
+    gy[y * P + x] = cy + omega * (ny - cy);
+  }
+}
+
+__global__ void f64_to_f32_kernel(const double* a, float* b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// area and face lengths (left, right, up, down) of every cell, AoS, from the
+// FP32 vertices (shaders/grid.js:128-147)
+__global__ void metrics_kernel(const float* vx, const float* vy, int M, int N, int P, float* area,
+                               float* flen) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  const int r = (x + 1 == M) ? 0 : x + 1;
+  const float ax = vx[y * P + x], ay = vy[y * P + x];
+  const float bx = vx[y * P + r], by = vy[y * P + r];
+  const float cx = vx[(y + 1) * P + x], cy = vy[(y + 1) * P + x];
+  const float dx = vx[(y + 1) * P + r], dy = vy[(y + 1) * P + r];
+  area[y * M + x] = 0.5f * fabsf((dx - ax) * (cy - by) - (cx - bx) * (dy - ay));
+  float* f = flen + ((size_t)y * M + x) * 4;
+  float ex, ey;
+  ex = ax - cx; ey = ay - cy; f[0] = sqrtf(ex * ex + ey * ey);
+  ex = bx - dx; ey = by - dy; f[1] = sqrtf(ex * ex + ey * ey);
+  ex = cx - dx; ey = cy - dy; f[2] = sqrtf(ex * ex + ey * ey);
+  ex = ax - bx; ey = ay - by; f[3] = sqrtf(ex * ex + ey * ey);
+}
+
+// --------------------------------------------------------------------------
+// wave speeds
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float nan_max2(float m, float x) { return (x > m || x != x) ? x : m; }
+
+__global__ void wave_speed_kernel(const float* q, const float* vx, const float* vy,
+                                  const DevParams* par, int M, int N, int P, int PS, float* ws,
+                                  float* part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float w = 0.0f;
+  if (i < M * N) {
+    const int y = i / M, x = i - y * M;
+    const int r = (x + 1 == M) ? 0 : x + 1;
+    const float ax = vx[y * P + x], ay = vy[y * P + x];
+    const float bx = vx[y * P + r], by = vy[y * P + r];
+    const float cx = vx[(y + 1) * P + x], cy = vy[(y + 1) * P + x];
+    const float dx = vx[(y + 1) * P + r], dy = vy[(y + 1) * P + r];
+    const float area = 0.5f * fabsf((dx - ax) * (cy - by) - (cx - bx) * (dy - ay));
+    float ex, ey, fl[4];
+    ex = ax - cx; ey = ay - cy; fl[0] = sqrtf(ex * ex + ey * ey);
+    ex = bx - dx; ey = by - dy; fl[1] = sqrtf(ex * ex + ey * ey);
+    ex = cx - dx; ey = cy - dy; fl[2] = sqrtf(ex * ex + ey * ey);
+    ex = ax - bx; ey = ay - by; fl[3] = sqrtf(ex * ex + ey * ey);
+    const int o = y * P + x;
+    const float rho = q[o];
+    const float u = q[PS + o] / rho, v = q[2 * PS + o] / rho;
+    const float vv = u * u + v * v;
+    const float p = (q[3 * PS + o] - (0.5f * rho) * vv) * (par->gamma - 1.0f);
+    const float T = p / rho;
+    const float c = sqrtf(par->gamma * T);
+    const float speed = sqrtf(vv);
+    const float S = ((fl[0] + fl[1]) + fl[2]) + fl[3];
+    w = ((speed + c) * S) / (2.0f * area);
+    if (ws) ws[i] = w;
+  }
+  __shared__ float sm[256];
+  sm[threadIdx.x] = w;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = nan_max2(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void reduce_max_kernel(const float* part, int n, float* out) {
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = nan_max2(m, part[i]);
+  __shared__ float sm[256];
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = nan_max2(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sm[0];
+}
+
+// sum of FP64 partials in index order (one thread: deterministic and the
+// partial counts are small)
+__global__ void reduce_sum_f64_kernel(const double* part, int n, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += part[i];
+  *out = sqrt(s);
+}
+
+// --------------------------------------------------------------------------
+// forces (graphs.js:16-101): per-segment p*l*(-n) from row-1 pressure,
+// summed in FP64 in index order
+// --------------------------------------------------------------------------
+__global__ void forces_kernel(const float* q, const float* vx, const float* vy,
+                              const DevParams* par, int M, int PS, double ref_len, double* seg,
+                              double* out, double* cp) {
+  const double Vx = par->vx, Vy = par->vy;
+  const double qinf = 0.5 * (double)par->inRho * (Vx * Vx + Vy * Vy);
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    const float rho = q[i];
+    const float u = q[PS + i] / rho, v = q[2 * PS + i] / rho;
+    const float p = (q[3 * PS + i] - (0.5f * rho) * (u * u + v * v)) * (par->gamma - 1.0f);
+    const int r = (i + 1 == M) ? 0 : i + 1;
+    const float ex = vx[r] - vx[i], ey = vy[r] - vy[i];
+    const float fl = sqrtf(ex * ex + ey * ey);
+    const float nx = (vy[r] - vy[i]) / fl, ny = (vx[i] - vx[r]) / fl;
+    const float pf = p * fl;
+    seg[2 * i] = (double)(pf * -nx);
+    seg[2 * i + 1] = (double)(pf * -ny);
+    if (cp) cp[i] = ((double)p - (double)par->inP) / qinf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double fx = 0.0, fy = 0.0;
+    for (int i = 0; i < M; ++i) {
+      fx += seg[2 * i];
+      fy += seg[2 * i + 1];
+    }
+    const double vn = sqrt(Vx * Vx + Vy * Vy);
+    const double ux = Vx / vn, uy = Vy / vn;
+    out[0] = fx;
+    out[1] = fy;
+    out[2] = (fx * -uy + fy * ux) / (qinf * ref_len);
+    out[3] = (fx * ux + fy * uy) / (qinf * ref_len);
+  }
+}
+
+// --------------------------------------------------------------------------
+// layouts
+// --------------------------------------------------------------------------
+// reference layout (N+3) x M x 4, rows 1..N -> SoA planes
+__global__ void aos_to_soa_kernel(const float* aos, float* q, int M, int N, int P, int PS) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  const float* s = aos + ((size_t)(y + 1) * M + x) * 4;
+  const int o = y * P + x;
+  q[o] = s[0];
+  q[PS + o] = s[1];
+  q[2 * PS + o] = s[2];
+  q[3 * PS + o] = s[3];
+}
+
+// SoA planes -> reference layout, ghost rows as the boundary pass makes them
+__global__ void soa_to_aos_kernel(const float* q, const float* vx, const float* vy,
+                                  const DevParams* par, int M, int N, int P, int PS, float* aos) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;  // 0 .. N+2
+  if (x >= M || y > N + 2) return;
+  const int r = (x + 1 == M) ? 0 : x + 1;
+  Cons g;
+  auto ld = [&](int j) {
+    const int o = j * P + x;
+    return Cons{q[o], q[PS + o], q[2 * PS + o], q[3 * PS + o]};
+  };
+  if (y >= 1 && y <= N) {
+    g = ld(y - 1);
+  } else if (y == 0) {
+    float nx, ny, len;
+    normal_exact(vx[x] - vx[r], vy[x] - vy[r], nx, ny, len);
+    g = wall_ghost(ld(0), nx, ny);
+  } else {
+    float nx, ny, len;
+    normal_exact(vx[N * P + x] - vx[N * P + r], vy[N * P + x] - vy[N * P + r], nx, ny, len);
+    g = far_ghost(*par, ld(N - 1), nx, ny, y == N + 2);
+  }
+  float* d = aos + ((size_t)y * M + x) * 4;
+  d[0] = g.r;
+  d[1] = g.mu;
+  d[2] = g.mv;
+  d[3] = g.e;
+}
+
+// residual planes -> N x M x 4
+__global__ void planes_to_aos_kernel(const float* q, int M, int N, int P, int PS, float* aos) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  const int o = y * P + x;
+  float* d = aos + ((size_t)y * M + x) * 4;
+  d[0] = q[o];
+  d[1] = q[PS + o];
+  d[2] = q[2 * PS + o];
+  d[3] = q[3 * PS + o];
+}
+
+// uniform state: rest (ausm.js:14-16) or inState
+__global__ void fill_state_kernel(float* q, const DevParams* par, int M, int N, int P, int PS,
+                                  int freestream) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  const int o = y * P + x;
+  if (freestream) {
+    q[o] = par->inState[0];
+    q[PS + o] = par->inState[1];
+    q[2 * PS + o] = par->inState[2];
+    q[3 * PS + o] = par->inState[3];
+  } else {
+    q[o] = par->inRho;
+    q[PS + o] = 0.0f;
+    q[2 * PS + o] = 0.0f;
+    q[3 * PS + o] = par->inP / (par->gamma - 1.0f);
+  }
+}
+
+// --------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------
+static dim3 rows_grid(int M, int rows) { return dim3((M + 255) / 256, rows); }
+
+hipError_t launch_grid_interp(double* gx, double* gy, int M, int N, int P, hipStream_t s) {
+  if (N < 2) return hipSuccess;
+  hipLaunchKernelGGL(grid_interp_kernel, rows_grid(M, N - 1), dim3(256), 0, s, gx, gy, M, N, P);
+  return hipGetLastError();
+}
+hipError_t launch_grid_jacobi(const double* gx, const double* gy, double* ox, double* oy, int M,
+                              int N, int P, hipStream_t s) {
+  hipLaunchKernelGGL(grid_jacobi_kernel, rows_grid(M, N + 1), dim3(256), 0, s, gx, gy, ox, oy, M, N,
+                     P);
+  return hipGetLastError();
+}
+hipError_t launch_grid_sor(double* gx, double* gy, int M, int N, int P, int color, double omega,
+                           hipStream_t s) {
+  if (N < 2) return hipSuccess;
+  hipLaunchKernelGGL(grid_sor_kernel, rows_grid(M, N - 1), dim3(256), 0, s, gx, gy, M, N, P, color,
+                     omega);
+  return hipGetLastError();
+}
+hipError_t launch_f64_to_f32(const double* a, float* b, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+  return hipGetLastError();
+}
+hipError_t launch_metrics(const float* vx, const float* vy, int M, int N, int P, float* area,
+                          float* flen, hipStream_t s) {
+  hipLaunchKernelGGL(metrics_kernel, rows_grid(M, N), dim3(256), 0, s, vx, vy, M, N, P, area, flen);
+  return hipGetLastError();
+}
+int wave_speed_blocks(int M, int N) { return (M * N + 255) / 256; }
+hipError_t launch_wave_speed(const float* q, const float* vx, const float* vy, const DevParams* par,
+                             int M, int N, int P, int PS, float* ws, float* part, hipStream_t s) {
+  hipLaunchKernelGGL(wave_speed_kernel, dim3(wave_speed_blocks(M, N)), dim3(256), 0, s, q, vx, vy,
+                     par, M, N, P, PS, ws, part);
+  return hipGetLastError();
+}
+hipError_t launch_reduce_max(const float* part, int n, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(256), 0, s, part, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_reduce_sum_f64(const double* part, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_sum_f64_kernel, dim3(1), dim3(64), 0, s, part, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_forces(const float* q, const float* vx, const float* vy, const DevParams* par,
+                         int M, int PS, double ref_len, double* seg, double* out, double* cp,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(256), 0, s, q, vx, vy, par, M, PS, ref_len, seg,
+                     out, cp);
+  return hipGetLastError();
+}
+hipError_t launch_aos_to_soa(const float* aos, float* q, int M, int N, int P, int PS,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(aos_to_soa_kernel, rows_grid(M, N), dim3(256), 0, s, aos, q, M, N, P, PS);
+  return hipGetLastError();
+}
+hipError_t launch_soa_to_aos(const float* q, const float* vx, const float* vy,
+                             const DevParams* par, int M, int N, int P, int PS, float* aos,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(soa_to_aos_kernel, rows_grid(M, N + 3), dim3(256), 0, s, q, vx, vy, par, M, N,
+                     P, PS, aos);
+  return hipGetLastError();
+}
+hipError_t launch_planes_to_aos(const float* q, int M, int N, int P, int PS, float* aos,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(planes_to_aos_kernel, rows_grid(M, N), dim3(256), 0, s, q, M, N, P, PS, aos);
+  return hipGetLastError();
+}
+hipError_t launch_fill_state(float* q, const DevParams* par, int M, int N, int P, int PS,
+                             int freestream, hipStream_t s) {
+  hipLaunchKernelGGL(fill_state_kernel, rows_grid(M, N), dim3(256), 0, s, q, par, M, N, P, PS,
+                     freestream);
+  return hipGetLastError();
+}
+
+}  // namespace ausm

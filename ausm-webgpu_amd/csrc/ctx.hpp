@@ -1,0 +1,53 @@
+// Single-instance solver context (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <vector>
+
+#include "device_math.hpp"
+#include "kernels.hpp"
+
+namespace ausm {
+
+struct GraphEntry {
+  int ty, steps, n_ws_in;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+}  // namespace ausm
+
+struct ausm_ctx {
+  int device = 0;
+  int M = 0, N = 0, P = 0, PS = 0;
+  hipStream_t stream = nullptr;
+  ausm::DevParams hpar{};
+  ausm::DevParams* dpar = nullptr;
+  float* q[3] = {nullptr, nullptr, nullptr};  // Q^n, Q1, Q2 (SoA planes)
+  float* vx = nullptr;                        // FP32 vertices used by the solver
+  float* vy = nullptr;
+  double* g64[4] = {nullptr, nullptr, nullptr, nullptr};  // FP64 grid ping-pong (x,y,x',y')
+  int g_cur = -1;                                          // 0 or 2: which pair holds the grid
+  float* ws_part = nullptr;   // per-workgroup max wave speeds
+  double* res_part = nullptr; // per-workgroup density-residual sums of squares
+  float* scal_f = nullptr;    // [0] max wave speed of the current dt, [1..] scratch
+  double* scal_d = nullptr;   // [0] residual norm, [2..5] forces
+  float* stage_buf = nullptr; // AoS staging, (N+3) x M x 4 floats
+  double* seg = nullptr;      // per-segment forces (2M) + cp (M)
+  void* pinned = nullptr;     // small pinned read-back area
+  int max_parts = 0;
+  int n_ws_valid = 0;
+  bool ws_fresh = false;
+  bool has_grid = false, has_state = false;
+  int scheme = 0, limiter = 0, math = 0, refresh = 1, tile = 0;
+  bool use_graphs = true;
+  long long steps = 0;
+  std::vector<ausm::GraphEntry> graphs;
+};
+
+namespace ausm {
+int set_error(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+StageArgs stage_args(ausm_ctx* c, int ty, int stage);
+}  // namespace ausm

@@ -1,0 +1,445 @@
+// Device-side arithmetic of the solver step: primitive conversion, MUSCL
+// reconstruction, the three AUSM-family face fluxes and the boundary ghost
+// states.
+//
+// Two variants of the face flux:
+//  * Exact — follows the reference WGSL (src/shaders/ausm.js) expression by
+//    expression: IEEE-rounded division and sqrt, no FMA contraction (the
+//    translation unit is built with -ffp-contract=off), the same operand
+//    order.  WGSL built-ins are given fixed definitions: dot sums left to
+//    right, normalize(v) = v/length(v), mix(a,b,t) = a*(1-t)+b*t,
+//    reflect(e,n) = e-(2*dot(n,e))*n, step(e,x) = x>=e, sign(0) = 0.
+//  * Fast — the same scheme rearranged for CDNA4: shared reciprocals
+//    (v_rcp_f32), hardware sqrt, one reciprocal per limiter ratio pair and
+//    FMA contraction.  It differs from Exact by a few ulp per operation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ausm {
+
+enum { SCHEME_SLAU2 = 0, SCHEME_SLAU = 1, SCHEME_AUSMUP = 2 };
+enum { LIM_VANALBADA = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_NONE = 3 };
+
+// Solver parameters resident in device memory so that captured graphs pick
+// up changes without being rebuilt (the reference's Uniforms buffer,
+// src/util.js:2-27).
+struct DevParams {
+  float gamma, inRho, inP, vx, vy;
+  float inState[4];
+  float maxdt, cfl, Kp, Ku;
+  // derived in double on the host, used by the fast path only
+  float gm1;       // gamma - 1
+  float ggm1;      // gamma / (gamma - 1)
+  float minf_coef; // |V|^2 / (gamma * p_inf)
+  float astar_coef; // 2 (gamma - 1) / (gamma + 1)
+};
+
+struct Prim {
+  float r, u, v, p;
+};
+struct Cons {
+  float r, mu, mv, e;
+};
+struct Flux {
+  float f0, f1, f2, f3;
+};
+
+__device__ __forceinline__ float wstep(float edge, float x) { return x >= edge ? 1.0f : 0.0f; }
+__device__ __forceinline__ float wsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+__device__ __forceinline__ float wsat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+__device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// toPrimitive (ausm.js:145-150)
+__device__ __forceinline__ Prim to_prim_exact(Cons q, float gamma) {
+  Prim w;
+  w.r = q.r;
+  w.u = q.mu / q.r;
+  w.v = q.mv / q.r;
+  const float d = w.u * w.u + w.v * w.v;
+  const float p = (q.e - (0.5f * q.r) * d) * (gamma - 1.0f);
+  w.p = fmaxf(1e-7f, p);
+  return w;
+}
+
+__device__ __forceinline__ Prim to_prim_fast(Cons q, float gm1) {
+#pragma clang fp contract(fast)
+  Prim w;
+  const float inv = __builtin_amdgcn_rcpf(q.r);
+  w.r = q.r;
+  w.u = q.mu * inv;
+  w.v = q.mv * inv;
+  const float p = (q.e - 0.5f * (q.mu * w.u + q.mv * w.v)) * gm1;
+  w.p = fmaxf(1e-7f, p);
+  return w;
+}
+
+// face unit normal: normalize(d) rotated by +90 degrees; also returns |d|
+// (the face length the residual uses, shaders/grid.js:143-146)
+__device__ __forceinline__ void normal_exact(float dx, float dy, float& nx, float& ny, float& len) {
+  len = sqrtf(dx * dx + dy * dy);
+  const float tx = dx / len, ty = dy / len;
+  nx = -ty;
+  ny = tx;
+}
+__device__ __forceinline__ void normal_fast(float dx, float dy, float& nx, float& ny, float& len) {
+#pragma clang fp contract(fast)
+  const float dd = dx * dx + dy * dy;
+  const float rs = __builtin_amdgcn_rsqf(dd);
+  len = dd * rs;
+  nx = -dy * rs;
+  ny = dx * rs;
+}
+
+// ---------------------------------------------------------------------------
+// Exact face flux (ausm.js:183-393)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float van_albada_exact(float r) {
+  const float r2 = r * r;
+  return (r2 + r) / (r2 + 1.0f);
+}
+__device__ __forceinline__ float van_leer_exact(float r) {
+  const float a = fabsf(r);
+  return (r + a) / (1.0f + a);
+}
+__device__ __forceinline__ float minmod_exact(float a, float b) {
+  return (0.5f * (wsign(a) + wsign(b))) * fminf(fabsf(a), fabsf(b));
+}
+
+// MUSCL on one primitive component (ausm.js:240-249)
+__device__ __forceinline__ void muscl_exact(int limiter, float wl2, float wl, float wr, float wr2,
+                                            float& L, float& Rr) {
+  const float eps = 1e-6f;
+  const float dL = wl - wl2;
+  const float dF = wr - wl;
+  const float dR = wr2 - wr;
+  if (limiter == LIM_MINMOD) {
+    L = wl + 0.5f * minmod_exact(dL, dF);
+    Rr = wr - 0.5f * minmod_exact(dF, dR);
+  } else {
+    const float rL = (dL * dF + eps) / (dF * dF + eps);
+    const float rR = (dF * dR + eps) / (dF * dF + eps);
+    const float phiL = limiter == LIM_VANLEER ? van_leer_exact(rL) : van_albada_exact(rL);
+    const float phiR = limiter == LIM_VANLEER ? van_leer_exact(rR) : van_albada_exact(rR);
+    L = wl + (0.5f * phiL) * dF;
+    Rr = wr - (0.5f * phiR) * dF;
+  }
+}
+
+template <int SCHEME>
+__device__ __forceinline__ Flux face_flux_exact(const DevParams& P, int limiter, Prim a, Prim b,
+                                                Prim c, Prim d, float nx, float ny) {
+  // a = W_L2, b = W_L, c = W_R, d = W_R2
+  const float gamma = P.gamma;
+  const float gammaMinus1 = gamma - 1.0f;
+  Prim L = b, R = c;
+  if (limiter != LIM_NONE) {
+    muscl_exact(limiter, a.r, b.r, c.r, d.r, L.r, R.r);
+    muscl_exact(limiter, a.u, b.u, c.u, d.u, L.u, R.u);
+    muscl_exact(limiter, a.v, b.v, c.v, d.v, L.v, R.v);
+    muscl_exact(limiter, a.p, b.p, c.p, d.p, L.p, R.p);
+  }
+  const float u_L = L.u * nx + L.v * ny;
+  const float u_R = R.u * nx + R.v * ny;
+  const float vvL = L.u * L.u + L.v * L.v;
+  const float vvR = R.u * R.u + R.v * R.v;
+  const float h_L = (L.p / (gammaMinus1 * L.r) + 0.5f * vvL) + L.p / L.r;
+  const float h_R = (R.p / (gammaMinus1 * R.r) + 0.5f * vvR) + R.p / R.r;
+  float mdot, P_i;
+  if constexpr (SCHEME == SCHEME_AUSMUP) {
+    // ausm.js:273-327
+    const float M_inf2 = (P.vx * P.vx + P.vy * P.vy) / ((gamma * P.inP) / L.r);
+    const float sigma = 1.0f;
+    const float h_t = 0.5f * (h_L + h_R);
+    const float aStar = sqrtf(fmaxf(1e-6f, ((2.0f * gammaMinus1) / (gamma + 1.0f)) * h_t));
+    const float aHat_L = (aStar * aStar) / fmaxf(u_L, aStar);
+    const float aHat_R = (aStar * aStar) / fmaxf(-u_R, aStar);
+    const float a_i = fminf(aHat_L, aHat_R);
+    const float M_L = u_L / a_i;
+    const float M_R = u_R / a_i;
+    const float a_i2 = a_i * a_i;
+    const float Mbar2 = (u_L * u_L + u_R * u_R) / (a_i2 * 2.0f);
+    const float M_o2 = fminf(1.0f, fmaxf(Mbar2, M_inf2));
+    const float f_a = M_o2 * (2.0f - M_o2);
+    const float rho_i = 0.5f * (L.r + R.r);
+    const float Mp1L = M_L + 1.0f, Mp1R = M_R + 1.0f;
+    const float Mm1L = M_L - 1.0f, Mm1R = M_R - 1.0f;
+    const float M2pL = (0.25f * Mp1L) * Mp1L, M2pR = (0.25f * Mp1R) * Mp1R;
+    const float M2mL = (-0.25f * Mm1L) * Mm1L, M2mR = (-0.25f * Mm1R) * Mm1R;
+    const float M4subL = M2pL * (1.0f + 2.0f * M2mL);
+    const float M4subR = M2mR * (1.0f - 2.0f * M2pR);
+    const float supL = wstep(1.0f, fabsf(M_L)), supR = wstep(1.0f, fabsf(M_R));
+    const float M4L = wmix(M4subL, fmaxf(0.0f, M_L), supL);
+    const float M4R = wmix(M4subR, fminf(0.0f, M_R), supR);
+    float kp = P.Kp / f_a;
+    kp = kp * fmaxf(1.0f - sigma * Mbar2, 0.0f);
+    kp = kp * (R.p - L.p);
+    kp = kp / (rho_i * a_i2);
+    const float M_i = (M4L + M4R) - kp;
+    const float alpha4 = 0.75f * (-4.0f + (5.0f * f_a) * f_a);
+    const float P5subL = M2pL * ((2.0f - M_L) + (alpha4 * M_L) * M2mL);
+    const float P5subR = M2mR * ((-2.0f - M_R) - (alpha4 * M_R) * M2pR);
+    const float P5L = wmix(P5subL, wstep(0.0f, M_L), supL);
+    const float P5R = wmix(P5subR, wstep(0.0f, -M_R), supR);
+    float ku = P.Ku * P5L;
+    ku = ku * P5R;
+    ku = ku * (L.r + R.r);
+    ku = ku * f_a;
+    ku = ku * a_i;
+    ku = ku * (u_R - u_L);
+    P_i = (P5L * L.p + P5R * R.p) - ku;
+    mdot = (a_i * M_i) * (M_i > 0.0f ? L.r : R.r);
+  } else {
+    // ausm.js:348-384
+    const float a_L = sqrtf((gamma * fmaxf(L.p, 1e-10f)) / fmaxf(L.r, 1e-10f));
+    const float a_R = sqrtf((gamma * fmaxf(R.p, 1e-10f)) / fmaxf(R.r, 1e-10f));
+    const float a_i = (a_L + a_R) * 0.5f;
+    const float M_L = u_L / a_i;
+    const float M_R = u_R / a_i;
+    const float Uarc = sqrtf(0.5f * (vvL + vvR));
+    const float Marc = fminf(1.0f, Uarc / a_i);
+    const float X = (1.0f - Marc) * (1.0f - Marc);
+    const float qL = M_L + 1.0f, qR = M_R - 1.0f;
+    const float sqL = (0.5f * qL) * qL, sqR = (0.5f * qR) * qR;
+    const float supL = wstep(1.0f, fabsf(M_L)), supR = wstep(1.0f, fabsf(M_R));
+    const float fpp = wsat(0.5f * wmix(sqL * (2.0f - 1.0f * M_L), 1.0f + 1.0f * wsign(M_L), supL));
+    const float fpm =
+        wsat(0.5f * wmix(sqR * (2.0f - -1.0f * M_R), 1.0f + -1.0f * wsign(M_R), supR));
+    if constexpr (SCHEME == SCHEME_SLAU2) {
+      const float t1 = (fpp - fpm) * (L.p - R.p);
+      const float t2 = ((Uarc * ((fpp + fpm) - 1.0f)) * (L.r + R.r)) * a_i;
+      P_i = 0.5f * ((t1 + (L.p + R.p)) + t2);
+    } else {
+      const float t1 = (fpp - fpm) * (L.p - R.p);
+      const float t2 = (1.0f + (1.0f - X) * ((fpp + fpm) - 1.0f)) * (L.p + R.p);
+      P_i = 0.5f * (t1 + t2);
+    }
+    const float delta_p = R.p - L.p;
+    const float ubar_n = (L.r * fabsf(u_L) + R.r * fabsf(u_R)) / (L.r + R.r);
+    const float g = wsat(-M_L) * wsat(M_R);
+    const float ubp = wmix(ubar_n, fabsf(u_L), g);
+    const float ubm = wmix(ubar_n, fabsf(u_R), g);
+    mdot = 0.5f * ((L.r * (u_L + ubp) + R.r * (u_R - ubm)) * (1.0f - g) - (X / a_i) * delta_p);
+  }
+  const bool left = mdot > 0.0f;
+  Flux F;
+  F.f0 = 1.0f * mdot + 0.0f;
+  F.f1 = (left ? L.u : R.u) * mdot + nx * P_i;
+  F.f2 = (left ? L.v : R.v) * mdot + ny * P_i;
+  F.f3 = (left ? h_L : h_R) * mdot + 0.0f;
+  return F;
+}
+
+// ---------------------------------------------------------------------------
+// Fast face flux: the same scheme with shared reciprocals and contraction.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void muscl_fast(int limiter, float wl2, float wl, float wr, float wr2,
+                                           float& L, float& Rr) {
+#pragma clang fp contract(fast)
+  const float eps = 1e-6f;
+  const float dL = wl - wl2;
+  const float dF = wr - wl;
+  const float dR = wr2 - wr;
+  if (limiter == LIM_MINMOD) {
+    const float mL = (dL * dF > 0.0f) ? copysignf(fminf(fabsf(dL), fabsf(dF)), dF) : 0.0f;
+    const float mR = (dF * dR > 0.0f) ? copysignf(fminf(fabsf(dF), fabsf(dR)), dF) : 0.0f;
+    L = wl + 0.5f * mL;
+    Rr = wr - 0.5f * mR;
+  } else if (limiter == LIM_VANLEER) {
+    // phi(r) = (r+|r|)/(1+|r|) with r = n/d, d > 0:  (n+|n|)/(d+|n|)
+    const float d = dF * dF + eps;
+    const float nL = dL * dF + eps, nR = dF * dR + eps;
+    const float phiL = (nL + fabsf(nL)) * __builtin_amdgcn_rcpf(d + fabsf(nL));
+    const float phiR = (nR + fabsf(nR)) * __builtin_amdgcn_rcpf(d + fabsf(nR));
+    L = wl + 0.5f * phiL * dF;
+    Rr = wr - 0.5f * phiR * dF;
+  } else {
+    // van Albada phi(r) = (r^2+r)/(r^2+1) with r = n/d:  n(n+d)/(n^2+d^2)
+    const float d = dF * dF + eps;
+    const float d2 = d * d;
+    const float nL = dL * dF + eps, nR = dF * dR + eps;
+    const float phiL = nL * (nL + d) * __builtin_amdgcn_rcpf(nL * nL + d2);
+    const float phiR = nR * (nR + d) * __builtin_amdgcn_rcpf(nR * nR + d2);
+    L = wl + 0.5f * phiL * dF;
+    Rr = wr - 0.5f * phiR * dF;
+  }
+}
+
+template <int SCHEME>
+__device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, Prim a, Prim b,
+                                               Prim c, Prim d, float nx, float ny) {
+#pragma clang fp contract(fast)
+  const float gamma = P.gamma;
+  Prim L = b, R = c;
+  if (limiter != LIM_NONE) {
+    muscl_fast(limiter, a.r, b.r, c.r, d.r, L.r, R.r);
+    muscl_fast(limiter, a.u, b.u, c.u, d.u, L.u, R.u);
+    muscl_fast(limiter, a.v, b.v, c.v, d.v, L.v, R.v);
+    muscl_fast(limiter, a.p, b.p, c.p, d.p, L.p, R.p);
+  }
+  const float u_L = L.u * nx + L.v * ny;
+  const float u_R = R.u * nx + R.v * ny;
+  const float vvL = L.u * L.u + L.v * L.v;
+  const float vvR = R.u * R.u + R.v * R.v;
+  const float rcL = fmaxf(L.r, 1e-10f), rcR = fmaxf(R.r, 1e-10f);
+  const float iclL = __builtin_amdgcn_rcpf(rcL), iclR = __builtin_amdgcn_rcpf(rcR);
+  const float irL = L.r > 1e-10f ? iclL : __builtin_amdgcn_rcpf(L.r);
+  const float irR = R.r > 1e-10f ? iclR : __builtin_amdgcn_rcpf(R.r);
+  const float h_L = P.ggm1 * (L.p * irL) + 0.5f * vvL;
+  const float h_R = P.ggm1 * (R.p * irR) + 0.5f * vvR;
+  float mdot, P_i;
+  if constexpr (SCHEME == SCHEME_AUSMUP) {
+    const float M_inf2 = P.minf_coef * L.r;
+    const float h_t = 0.5f * (h_L + h_R);
+    const float aStar = __builtin_amdgcn_sqrtf(fmaxf(1e-6f, P.astar_coef * h_t));
+    const float as2 = aStar * aStar;
+    const float aHat_L = as2 * __builtin_amdgcn_rcpf(fmaxf(u_L, aStar));
+    const float aHat_R = as2 * __builtin_amdgcn_rcpf(fmaxf(-u_R, aStar));
+    const float a_i = fminf(aHat_L, aHat_R);
+    const float ia = __builtin_amdgcn_rcpf(a_i);
+    const float M_L = u_L * ia;
+    const float M_R = u_R * ia;
+    const float a_i2 = a_i * a_i;
+    const float Mbar2 = (u_L * u_L + u_R * u_R) * (0.5f * ia * ia);
+    const float M_o2 = fminf(1.0f, fmaxf(Mbar2, M_inf2));
+    const float f_a = M_o2 * (2.0f - M_o2);
+    const float rho_i = 0.5f * (L.r + R.r);
+    const float Mp1L = M_L + 1.0f, Mp1R = M_R + 1.0f;
+    const float Mm1L = M_L - 1.0f, Mm1R = M_R - 1.0f;
+    const float M2pL = 0.25f * Mp1L * Mp1L, M2pR = 0.25f * Mp1R * Mp1R;
+    const float M2mL = -0.25f * Mm1L * Mm1L, M2mR = -0.25f * Mm1R * Mm1R;
+    const bool supL = fabsf(M_L) >= 1.0f, supR = fabsf(M_R) >= 1.0f;
+    const float M4L = supL ? fmaxf(0.0f, M_L) : M2pL * (1.0f + 2.0f * M2mL);
+    const float M4R = supR ? fminf(0.0f, M_R) : M2mR * (1.0f - 2.0f * M2pR);
+    float M_i = M4L + M4R;
+    if (P.Kp != 0.0f) {
+      const float kp = P.Kp * fmaxf(1.0f - Mbar2, 0.0f) * (R.p - L.p) *
+                       __builtin_amdgcn_rcpf(f_a * rho_i * a_i2);
+      M_i -= kp;
+    }
+    const float alpha4 = 0.75f * (-4.0f + 5.0f * f_a * f_a);
+    const float P5L = supL ? (M_L >= 0.0f ? 1.0f : 0.0f)
+                           : M2pL * ((2.0f - M_L) + alpha4 * M_L * M2mL);
+    const float P5R = supR ? (-M_R >= 0.0f ? 1.0f : 0.0f)
+                           : M2mR * ((-2.0f - M_R) - alpha4 * M_R * M2pR);
+    const float ku = P.Ku * P5L * P5R * (L.r + R.r) * f_a * a_i * (u_R - u_L);
+    P_i = (P5L * L.p + P5R * R.p) - ku;
+    mdot = a_i * M_i * (M_i > 0.0f ? L.r : R.r);
+  } else {
+    const float a_L = __builtin_amdgcn_sqrtf(gamma * fmaxf(L.p, 1e-10f) * iclL);
+    const float a_R = __builtin_amdgcn_sqrtf(gamma * fmaxf(R.p, 1e-10f) * iclR);
+    const float a_i = (a_L + a_R) * 0.5f;
+    const float ia = __builtin_amdgcn_rcpf(a_i);
+    const float M_L = u_L * ia;
+    const float M_R = u_R * ia;
+    const float Uarc = __builtin_amdgcn_sqrtf(0.5f * (vvL + vvR));
+    const float Marc = fminf(1.0f, Uarc * ia);
+    const float X = (1.0f - Marc) * (1.0f - Marc);
+    const bool supL = fabsf(M_L) >= 1.0f, supR = fabsf(M_R) >= 1.0f;
+    const float qL = M_L + 1.0f, qR = M_R - 1.0f;
+    const float fpp = wsat(supL ? 0.5f * (1.0f + wsign(M_L)) : 0.25f * qL * qL * (2.0f - M_L));
+    const float fpm = wsat(supR ? 0.5f * (1.0f - wsign(M_R)) : 0.25f * qR * qR * (2.0f + M_R));
+    if constexpr (SCHEME == SCHEME_SLAU2) {
+      P_i = 0.5f * ((fpp - fpm) * (L.p - R.p) + (L.p + R.p) +
+                    Uarc * ((fpp + fpm) - 1.0f) * (L.r + R.r) * a_i);
+    } else {
+      P_i = 0.5f * ((fpp - fpm) * (L.p - R.p) +
+                    (1.0f + (1.0f - X) * ((fpp + fpm) - 1.0f)) * (L.p + R.p));
+    }
+    const float delta_p = R.p - L.p;
+    const float ubar_n =
+        (L.r * fabsf(u_L) + R.r * fabsf(u_R)) * __builtin_amdgcn_rcpf(L.r + R.r);
+    const float g = wsat(-M_L) * wsat(M_R);
+    const float ubp = ubar_n + g * (fabsf(u_L) - ubar_n);
+    const float ubm = ubar_n + g * (fabsf(u_R) - ubar_n);
+    mdot = 0.5f * ((L.r * (u_L + ubp) + R.r * (u_R - ubm)) * (1.0f - g) - X * ia * delta_p);
+  }
+  const bool left = mdot > 0.0f;
+  Flux F;
+  F.f0 = mdot;
+  F.f1 = (left ? L.u : R.u) * mdot + nx * P_i;
+  F.f2 = (left ? L.v : R.v) * mdot + ny * P_i;
+  F.f3 = (left ? h_L : h_R) * mdot;
+  return F;
+}
+
+// ---------------------------------------------------------------------------
+// Ghost states (boundary pass, ausm.js:34-120).  Computed on the fly from the
+// first / last interior row instead of being stored.
+// ---------------------------------------------------------------------------
+
+// slip wall: reflect the velocity of interior row 1 about the body-segment
+// normal (ausm.js:57-64).  (nx, ny) = rot90(normalize(v_i - v_{i+1})).
+__device__ __forceinline__ Cons wall_ghost(Cons I, float nx, float ny) {
+  const float rho = I.r;
+  const float ux = I.mu / rho, uy = I.mv / rho;
+  const float k = 2.0f * (nx * ux + ny * uy);
+  const float gx = ux - k * nx, gy = uy - k * ny;
+  Cons g;
+  g.r = rho;
+  g.mu = gx * rho;
+  g.mv = gy * rho;
+  g.e = I.e;
+  return g;
+}
+
+// far field from interior row N (ausm.js:65-117).  second = the outer ghost
+// row (N+2), which extrapolates linearly on outflow.
+__device__ __forceinline__ Cons far_ghost(const DevParams& P, Cons I, float nx, float ny,
+                                          bool second) {
+  const float gamma = P.gamma;
+  const float rho = I.r;
+  const float ux = I.mu / rho, uy = I.mv / rho;
+  const float bio = nx * P.vx + ny * P.vy;
+  Cons g;
+  if (bio < 0.0f) {
+    g.r = P.inState[0];
+    g.mu = P.inState[1];
+    g.mv = P.inState[2];
+    g.e = P.inState[3];
+    return g;
+  }
+  const float gm1inv = 1.0f / (gamma - 1.0f);
+  const float u2 = ux * ux + uy * uy;
+  const float Pr = (I.e - (0.5f * u2) * rho) * (gamma - 1.0f);
+  const float a2 = (gamma * Pr) / rho;
+  const float a2inf = (gamma * P.inP) / P.inRho;
+  const float M_int = u2 / a2;
+  const float M_inf2 = (P.vx * P.vx + P.vy * P.vy) / a2inf;
+  if (M_int < 1.0f && M_inf2 < 1.0f) {
+    const float un_int = ux * nx + uy * ny;
+    const float un_inf = bio;
+    const float a_int = sqrtf(a2);
+    const float a_inf = sqrtf(a2inf);
+    const float J_int = un_int + (2.0f * a_int) * gm1inv;
+    const float J_ext = un_inf - (2.0f * a_inf) * gm1inv;
+    const float un_g = 0.5f * (J_int + J_ext);
+    const float a_g = fmaxf(a_int * 0.1f, (0.25f * (J_int - J_ext)) * (gamma - 1.0f));
+    const float X = fmaxf(1e-6f, a_g / a_int);
+    const float X2 = X * X;
+    const float rho_g = fmaxf(1e-6f, ((rho * X2) * X2) * X);
+    const float dn = un_g - un_int;
+    const float vgx = dn * nx + ux, vgy = dn * ny + uy;
+    const float rhoE_g = fmaxf(1e-6f, ((rho_g * a_g) * a_g) / gamma) * gm1inv +
+                         (0.5f * rho_g) * (vgx * vgx + vgy * vgy);
+    g.r = rho_g;
+    g.mu = vgx * rho_g;
+    g.mv = vgy * rho_g;
+    g.e = rhoE_g;
+  } else if (M_inf2 > 1.0f) {
+    g.r = I.r;
+    g.mu = I.mu;
+    g.mv = I.mv;
+    g.e = P.inP / (gamma - 1.0f) + (0.5f * rho) * u2;
+  } else {
+    g = I;
+  }
+  if (second) {
+    g.r = 2.0f * g.r - I.r;
+    g.mu = 2.0f * g.mu - I.mu;
+    g.mv = 2.0f * g.mv - I.mv;
+    g.e = 2.0f * g.e - I.e;
+  }
+  return g;
+}
+
+}  // namespace ausm

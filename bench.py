@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit O-grid solver step on one MI355X.
+
+Workload (the single-instance configuration the headline metric is quoted
+on): NACA 2412 O-grid of 512 x 384 cells at the reference's default geometry
+(chord 0.6 at x = -0.7 inside the +-1.5 x +-1 rectangular far field,
+power-law 7/5 body spacing, 10^4 FP64 Jacobi grid sweeps on the GPU), FP32
+state, Mach 2.0, AoA 0, AUSM+-up with MUSCL (van Albada), TVD-RK3, CFL 0.5,
+uniform free-stream start.  A "step" is one TVD-RK3 time step of the whole
+grid (three fused stage kernels, dt refreshed every step).
+
+Prints one JSON line: cell-steps/s over K timed steps (inputs resident in
+HBM), the stage kernel's roofline position from HIP events around
+individual launches, and the CPU baseline (the FP32 oracle in the
+reference's unfused 15-pass structure, OpenMP over rows) on a bounded
+sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ausm-webgpu_amd"))
+
+ALG_BYTES_PER_CELL_STEP = 152.0  # 8 x 16 B state streams + 3 x 8 B vertex
+HBM_PEAK_GBPS = 8000.0
+METRIC = "cell-steps/s & % HBM roofline; batched converged solves/s + p50/p99 latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """HIP events on an explicit stream (the solver's own stream)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.L.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.L.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert self.L.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        return ms.value
+
+    def device_sync(self):
+        assert self.L.hipDeviceSynchronize() == 0
+
+
+def build_workload(ausm, M, N, grid_iters):
+    body = ausm.naca4_boundary(M, code=2412, size=0.3, x_offset=-0.7, exponent=1.4)
+    outer = ausm.rect_boundary(M, 1.5)
+    s = ausm.Solver(M, N, device=0)
+    s.update_grid_boundaries(body, outer)
+    t0 = time.perf_counter()
+    s.prepare_grid(iters=grid_iters)
+    t_grid = time.perf_counter() - t0
+    s.set_flow(2.0, 0.0)
+    s.set_scheme(ausm.AUSMUP, ausm.VANALBADA, 0.0, 0.75)
+    s.set_cfl(0.5, 1.0, 1)
+    return s, t_grid
+
+
+def time_steps(s, ev, steps, warmup):
+    s.prepare_state(freestream=True)
+    s.step(warmup, info=False)
+    s.sync()
+    ev.device_sync()
+    e0, e1 = ev.event(), ev.event()
+    stream = s.stream
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    s.step(steps, info=False)
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1)
+    s.sync()
+    wall = time.perf_counter() - t0
+    info = s.step(1)
+    return ms, wall, info
+
+
+def cpu_baseline(vtx, M, N, target_s=15.0):
+    """The FP32 oracle, unfused 15 passes per step, OpenMP over rows, on a
+    bounded number of steps of the same 512 x 384 workload."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import make_params
+
+    libp = os.path.join(ROOT, "oracle", "libbaseline.so")
+    if not os.path.exists(libp):
+        import subprocess
+
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    B = ctypes.CDLL(libp)
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    B.base32_run_steps.argtypes = [vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, vp, vp, vp]
+    B.base32_run_steps.restype = ctypes.c_float
+    B.base32_grid_finalize.argtypes = [ci, ci, vp, vp, vp]
+    B.base32_init_state.argtypes = [vp, ci, ci, ci, vp]
+    B.base32_max_wave_speed.argtypes = [vp, ci, ci, vp, vp, vp, vp]
+    B.base32_max_wave_speed.restype = ctypes.c_float
+    B.base32_num_threads.restype = ci
+    threads = B.base32_num_threads()
+    P = lambda a: a.ctypes.data_as(vp)
+    par = make_params(2.0, 0.0, cfl=0.5, maxdt=1.0)
+    v = np.ascontiguousarray(vtx, dtype=np.float32)
+    area = np.zeros((N, M), np.float32)
+    flen = np.zeros((N, M, 4), np.float32)
+    B.base32_grid_finalize(M, N, P(v), P(area), P(flen))
+    st = np.zeros((N + 3, M, 4), np.float32)
+    B.base32_init_state(P(par), M, N, 1, P(st))
+    s0, s1, s2 = st.copy(), st.copy(), st.copy()
+    cnt = ctypes.c_longlong(0)
+    mws = np.array([B.base32_max_wave_speed(P(par), M, N, P(s2), P(flen), P(area), None)],
+                   np.float32)
+
+    def run(n):
+        t0 = time.perf_counter()
+        B.base32_run_steps(P(par), 2, 0, M, N, P(v), P(flen), P(area), P(s0), P(s1), P(s2), n, 1,
+                           ctypes.byref(cnt), P(mws), None)
+        return time.perf_counter() - t0
+
+    run(1)  # warm-up
+    t1 = run(2) / 2
+    n = max(2, min(2000, int(target_s / max(t1, 1e-6))))
+    dt = run(n)
+    return {"value": M * N * n / dt, "unit": "cell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} steps of the 512x384 workload (ref32 oracle, 15 unfused passes per "
+                      f"step, OpenMP over rows, {threads} threads, {dt:.1f} s)",
+            "steps_per_s": n / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--grid-iters", type=int, default=10000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--math", default="fast", choices=["fast", "exact"])
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: the solver step has no multi-GPU path in this "
+              "build (batched solves shard by solve, one process per GPU; see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ausm
+
+    M, N = 512, 384
+    ev = HipEvents()
+    log(f"[bench] building {M}x{N} NACA 2412 grid ({args.grid_iters} FP64 Jacobi sweeps)")
+    s, t_grid = build_workload(ausm, M, N, args.grid_iters)
+    log(f"[bench] grid in {t_grid:.2f} s")
+
+    # exact-math reference line (shorter run)
+    s.set_math("exact")
+    ms_x, _, _ = time_steps(s, ev, max(1, args.steps // 10), min(args.warmup, 100))
+    exact_rate = M * N * max(1, args.steps // 10) / (ms_x / 1e3)
+    log(f"[bench] exact math: {exact_rate:.3e} cell-steps/s")
+
+    s.set_math(args.math)
+    log(f"[bench] timing {args.steps} steps ({args.math} math) after {args.warmup} warm-up")
+    ms, wall, info = time_steps(s, ev, args.steps, args.warmup)
+    rate = M * N * args.steps / (ms / 1e3)
+    step_ms = ms / args.steps
+    log(f"[bench] {rate:.3e} cell-steps/s, {step_ms * 1e3:.2f} us/step, wall {wall:.3f} s, "
+        f"res_l2 {info['res_l2']:.3e}")
+
+    # per-launch stage-kernel durations (HIP events around single launches)
+    ms3 = (ctypes.c_float * 3)()
+    ausm.lib.check(s._L.ausm_profile_stages(s.h, 200, ms3))
+    stage_ms = [ms3[0], ms3[1], ms3[2]]
+    avg_launch_s = sum(stage_ms) / 3 / 1e3
+    per_launch_bytes = ALG_BYTES_PER_CELL_STEP / 3 * M * N
+    achieved = per_launch_bytes / avg_launch_s / 1e9
+    log(f"[bench] stage kernels: {[round(x * 1e3, 2) for x in stage_ms]} us")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(s.vertices(), M, N)
+        log(f"[bench] CPU: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
+
+    whole_gbps = ALG_BYTES_PER_CELL_STEP * rate / 1e9
+    out = {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "NACA 2412 O-grid 512x384 (reference default geometry), Mach 2.0, "
+                               "AoA 0, AUSM+-up + MUSCL van Albada, TVD-RK3, CFL 0.5, "
+                               "free-stream start",
+                   "M": M, "N": N, "math": args.math, "dt_refresh_every": 1,
+                   "parallelism": "single GPU"},
+        "pct_hbm_roofline_whole_step": 100.0 * whole_gbps / HBM_PEAK_GBPS,
+        "steps_per_s": args.steps / (ms / 1e3),
+        "exact_math_cell_steps_per_s": exact_rate,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "ausm::stage_kernel (fused RK stage)",
+                     "stage_us": [x * 1e3 for x in stage_ms]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

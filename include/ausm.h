@@ -1,0 +1,205 @@
+/*
+ * ausm.h — C ABI of the MI355X-native explicit O-grid solver step.
+ *
+ * Drop-in replacement for the reference's WebGPU compute passes
+ * (huj31415/ausm-webgpu): grid + free stream + scheme/limiter/CFL in,
+ * conserved-state buffer out.  Plain C types only; every call returns an
+ * int status (AUSM_OK = 0, negative on error) and ausm_last_error() gives a
+ * thread-local message.  Host arrays passed in are copied, never retained.
+ *
+ * Array layouts are the reference's texture layouts, row-major with the i
+ * index (around the body) fastest:
+ *   vertices  (N+1) x M x 2   gridPoints  (reference src/main.js:110)
+ *   state     (N+3) x M x 4   state0/1/2  (src/main.js:117-119): row 0 is the
+ *             body ghost, rows 1..N the interior cells, rows N+1, N+2 the
+ *             far-field ghosts; (rho, rho*u, rho*v, rho*E)
+ *   residual  N x M x 4       (src/main.js:123)
+ *   area      N x M           (src/main.js:113)
+ *   face len  N x M x 4       (left, right, up, down)  (src/main.js:114)
+ *
+ * Threading: a context belongs to one host thread at a time; separate
+ * contexts may be used concurrently.  Each context owns one HIP stream and
+ * all of its device buffers.
+ */
+#ifndef AUSM_H
+#define AUSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AUSM_VERSION 1
+
+/* status codes */
+#define AUSM_OK 0
+#define AUSM_ERR_ARG (-1)       /* bad argument */
+#define AUSM_ERR_HIP (-2)       /* HIP runtime error */
+#define AUSM_ERR_NOMEM (-3)     /* device or host allocation failed */
+#define AUSM_ERR_ORDER (-4)     /* call out of order (no grid yet, ...) */
+#define AUSM_ERR_NONFINITE (-5) /* NaN/Inf detected in the solution */
+#define AUSM_ERR_FULL (-6)      /* batch: no free slot or memory */
+#define AUSM_ERR_NOTFOUND (-7)  /* batch: unknown solve id */
+
+/* flux schemes, in the order of the reference's `solvers` table
+ * (src/util.js:114-118) */
+enum ausm_scheme { AUSM_SLAU2 = 0, AUSM_SLAU = 1, AUSM_AUSMUP = 2 };
+
+/* MUSCL limiters.  VANALBADA is the one the reference wires
+ * (src/shaders/ausm.js:247-248); MINMOD and VANLEER are the reference's
+ * unwired limiter functions (ausm.js:159-166); NONE is uni.muscl = 0
+ * (first order, ausm.js:240). */
+enum ausm_limiter { AUSM_VANALBADA = 0, AUSM_MINMOD = 1, AUSM_VANLEER = 2, AUSM_NOLIMITER = 3 };
+
+/* arithmetic of the stage kernels: EXACT follows the reference WGSL
+ * expression by expression with IEEE division/sqrt and no contraction;
+ * FAST shares reciprocals, uses hardware rcp/sqrt and FMA contraction. */
+enum ausm_math { AUSM_MATH_EXACT = 0, AUSM_MATH_FAST = 1 };
+
+/* initial state: REST is the reference's prepareState (ausm.js:1-18, air at
+ * rest); FREESTREAM fills every cell with inState (main.js:783-784). */
+enum ausm_init { AUSM_INIT_REST = 0, AUSM_INIT_FREESTREAM = 1 };
+
+/* elliptic grid smoother: JACOBI is the reference's ping-pong sweep
+ * (src/shaders/grid.js:44-110); SOR is red-black over-relaxation of the same
+ * update (needs even M). */
+enum ausm_grid_method { AUSM_GRID_JACOBI = 0, AUSM_GRID_SOR = 1 };
+
+/* body-point spacing along the chord */
+enum ausm_spacing { AUSM_SPACING_POWER = 0, AUSM_SPACING_COSINE = 1 };
+
+typedef struct ausm_ctx ausm_ctx;
+
+typedef struct ausm_step_info {
+  float dt;             /* time step of the last step taken */
+  float max_wave_speed; /* max wave speed that dt is currently based on */
+  double res_l2;        /* FP64 L2 norm of the density residual L(Q^n) of the
+                           last step (stage 1), over all interior cells */
+  int nonfinite;        /* 1 if the max wave speed is not finite */
+  long long steps_done; /* steps taken on this context so far */
+} ausm_step_info;
+
+/* ---- errors / version ---------------------------------------------------- */
+const char* ausm_last_error(void);
+int ausm_version(void);
+
+/* ---- context (replaces requestDevice + the storage table, main.js:5-151,
+ *      util.js:41-74) ---------------------------------------------------- */
+int ausm_ctx_create(int device, int M, int N, ausm_ctx** out);
+int ausm_ctx_destroy(ausm_ctx* ctx);
+/* the context's HIP stream (hipStream_t), for callers that time it */
+int ausm_ctx_stream(ausm_ctx* ctx, void** stream);
+int ausm_sync(ausm_ctx* ctx);
+
+/* ---- body / far-field generators (src/grid.js:38-52, 232-255), FP64 ----- */
+/* NACA 4-digit section with maximum camber m, its position p and thickness
+ * t (NACA 2412: m=0.02, p=0.4, t=0.12), closed trailing edge; chord 2*size
+ * centred at x_offset; M points CCW from the trailing edge. */
+int ausm_boundary_naca4(double m, double p, double t, double size, double x_offset,
+                        double exponent, int spacing, int M, double* xy);
+int ausm_boundary_circle(double radius, int M, double* xy);
+int ausm_boundary_rect(double length_ratio, int M, double* xy);
+
+/* ---- grid (prepareGrid, main.js:658-707) -------------------------------- */
+/* interpolation + `iters` elliptic sweeps + finalize, in FP64 on the GPU;
+ * omega is used by SOR only. */
+int ausm_grid_from_boundary(ausm_ctx* ctx, const double* body_xy, const double* outer_xy,
+                            int iters, int method, double omega);
+int ausm_grid_set_vertices(ausm_ctx* ctx, const float* xy);
+int ausm_grid_get_vertices(ausm_ctx* ctx, float* xy);
+int ausm_grid_get_vertices64(ausm_ctx* ctx, double* xy);
+int ausm_grid_get_metrics(ausm_ctx* ctx, float* area, float* face_lengths);
+
+/* ---- parameters (Uniforms fields, src/util.js:2-27) --------------------- */
+/* free stream; inState = (rho, rho*vx, rho*vy, p/(gamma-1) + rho*|v|^2/2) is
+ * formed in double and stored as FP32, as main.js:780-784 does. */
+int ausm_set_freestream(ausm_ctx* ctx, double rho, double p, double vx, double vy,
+                        double gamma);
+int ausm_set_scheme(ausm_ctx* ctx, int scheme, int limiter, float K_p, float K_u);
+/* dt = min(maxdt, cfl / max wave speed); the wave speed is refreshed from
+ * the solution after every `steps_per_dt_refresh` steps (the reference
+ * refreshes once per frame, main.js:789-791). */
+int ausm_set_cfl(ausm_ctx* ctx, float cfl, float maxdt, int steps_per_dt_refresh);
+int ausm_set_math(ausm_ctx* ctx, int math);
+/* rows per wavefront tile of the stage kernel (0 = autotuned) */
+int ausm_set_tile(ausm_ctx* ctx, int rows_per_wave);
+
+/* ---- state and stepping (prepareState / render step loop,
+ *      main.js:631-655, 742-792) ------------------------------------------ */
+int ausm_init_state(ausm_ctx* ctx, int mode);
+int ausm_set_state(ausm_ctx* ctx, const float* q);
+/* interior rows as stored; ghost rows as the next boundary pass makes them */
+int ausm_get_state(ausm_ctx* ctx, float* q);
+int ausm_step(ausm_ctx* ctx, int nsteps, ausm_step_info* info);
+/* L(Q) of the current solution (boundary + fluxes + residual passes) */
+int ausm_eval_residual(ausm_ctx* ctx, float* res);
+/* per-cell wave speeds of the current solution (ausm.js:617-630) and their
+ * max (cflReduction, ausm.js:535-582); ws may be NULL */
+int ausm_wave_speeds(ausm_ctx* ctx, float* ws, float* max_ws);
+/* surface force from row-1 pressure (graphs.js:16-101), summed in FP64:
+ * out = (Fx, Fy, Cl, Cd); Cl and Cd are divided by q_inf * ref_length
+ * (ref_length = 1 gives the reference's un-normalised values).  cp (M
+ * values, optional) gets (p - p_inf)/q_inf per body segment. */
+int ausm_forces(ausm_ctx* ctx, double ref_length, double* out, double* cp);
+/* FP64 L2 norm of the density residual of the current solution */
+int ausm_residual_norm(ausm_ctx* ctx, double* l2);
+
+/* Timing hook, no counterpart in the reference (it times whole passes with
+ * timestamp queries, main.js:597-602): launches each of the three stage
+ * kernels `reps` times on the context's stream, each launch bracketed by HIP
+ * events, and returns the mean duration of each stage in ms.  The solution
+ * is left untouched (stage outputs go to the scratch buffers). */
+int ausm_profile_stages(ausm_ctx* ctx, int reps, float* ms3);
+
+/* ---- batched solves and continuous-batching service ---------------------
+ * No counterpart in the reference (it runs one grid).  A batch context
+ * holds many independent solves of different sizes resident in HBM and
+ * advances all active ones together with one persistent kernel per RK
+ * stage.  Descriptor fields mirror the single-instance setters above. */
+typedef struct ausm_solve_desc {
+  int M, N;              /* cells around / radial */
+  const float* vertices; /* (N+1) x M x 2, FP32 (e.g. from ausm_grid_get_vertices) */
+  double rho, p, vx, vy, gamma;
+  int scheme, limiter;
+  float K_p, K_u;
+  float cfl, maxdt;
+  int init_mode;         /* AUSM_INIT_* */
+  int max_steps;         /* stop after this many steps */
+  double res_drop;       /* converged when res_l2 <= res_drop * first res_l2 (0 = never) */
+  double ref_length;     /* for Cl/Cd */
+} ausm_solve_desc;
+
+typedef struct ausm_solve_status {
+  int state;        /* 0 = queued, 1 = running, 2 = converged, 3 = step cap, 4 = failed */
+  long long steps;  /* steps taken */
+  double res_l2;    /* last density-residual norm */
+  double res_l2_0;  /* first density-residual norm */
+  double cl, cd;    /* forces at completion (or at the last poll) */
+} ausm_solve_status;
+
+typedef struct ausm_batch ausm_batch;
+
+/* max_solves bounds the slots; hbm_budget_bytes (0 = free memory less a
+ * reserve) bounds admission by device memory. */
+int ausm_batch_create(int device, int max_solves, size_t hbm_budget_bytes, ausm_batch** out);
+int ausm_batch_destroy(ausm_batch* b);
+int ausm_batch_stream(ausm_batch* b, void** stream);
+int ausm_batch_set_math(ausm_batch* b, int math);
+/* queue a solve; it is admitted when a slot and memory are free */
+int ausm_batch_submit(ausm_batch* b, const ausm_solve_desc* desc, uint64_t* id);
+/* advance every running solve by `nsteps` steps (one persistent launch per
+ * RK stage per step), check convergence every `check_every` steps, retire
+ * finished solves and admit queued ones; returns the number still active or
+ * queued through *pending. */
+int ausm_batch_advance(ausm_batch* b, int nsteps, int check_every, int* pending);
+int ausm_batch_poll(ausm_batch* b, uint64_t id, ausm_solve_status* st);
+/* copy a solve's state (finished or running) in the reference layout */
+int ausm_batch_get_state(ausm_batch* b, uint64_t id, float* q);
+int ausm_batch_release(ausm_batch* b, uint64_t id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AUSM_H */

@@ -1,0 +1,337 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Bar:
+  * exact math: bit-for-bit equal to ref32 (the FP32 restatement of the
+    reference WGSL) for residuals, states after any number of steps, wave
+    speeds, forces, metrics and ghost rows, and the FP64 grid bit-for-bit
+    equal to ref64's grid;
+  * fast math: per-step residual within max-rel 1e-5 of ref32, where
+    max-rel = max|gpu - ref| / max|ref| per conserved component.
+Inputs: seeded free stream + 1 % sinusoidal perturbation of (rho, u, v, p) on
+NACA O-grids, and a shock-containing state from a supersonic run.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ausm
+from oracle_lib import (AUSMUP, MINMOD, NONE, SLAU, SLAU2, VANALBADA, VANLEER, Oracle, Solver,
+                        circle_boundary, make_params, naca4_boundary, naca4_digits, rect_boundary)
+
+pytestmark = pytest.mark.gpu
+
+SCHEMES = [SLAU2, SLAU, AUSMUP]
+LIMITERS = [VANALBADA, MINMOD, VANLEER, NONE]
+FAST_TOL = 1e-5
+
+
+def make_grid(M, N, code=2412, iters=200, radius=10.0, spacing=1, rect=False):
+    o = Oracle(64)
+    m, p, t = naca4_digits(code)
+    body = naca4_boundary(M, m, p, t, size=0.5, x_offset=0.0, expo=1.4, spacing=spacing)
+    outer = rect_boundary(M, 1.5) if rect else circle_boundary(M, radius)
+    return o.grid(body, outer, N, iters).astype(np.float32), body, outer
+
+
+def perturbed_state(par, M, N, seed, amp=0.01):
+    rng = np.random.default_rng(seed)
+    g = float(par[0])
+    rho0 = float(par[5])
+    u0, v0 = float(par[6]) / rho0, float(par[7]) / rho0
+    p0 = float(par[2])
+    jj, ii = np.meshgrid(np.arange(N + 3), np.arange(M), indexing="ij")
+    ph = rng.uniform(0, 2 * math.pi, 4)
+    k = rng.integers(1, 4, 4)
+    w = [1 + amp * np.sin(2 * math.pi * k[c] * ii / M + 0.37 * jj + ph[c]) for c in range(4)]
+    rho = rho0 * w[0]
+    u = u0 * w[1] + 0.05 * amp * np.cos(0.5 * jj)
+    v = v0 * w[2] + 0.05 * amp * np.sin(0.3 * jj + ii)
+    p = p0 * w[3]
+    st = np.stack([rho, rho * u, rho * v, p / (g - 1) + 0.5 * rho * (u * u + v * v)], axis=-1)
+    return st.astype(np.float32)
+
+
+def configure(s, fs, par, scheme, limiter, math_mode="exact", refresh=1, tile=0):
+    s.set_freestream(*fs)
+    s.set_scheme(scheme, limiter, float(par[11]), float(par[12]))
+    s.set_cfl(float(par[10]), float(par[9]), refresh)
+    s.set_math(math_mode)
+    s.set_tile(tile)
+
+
+def flow(mach, aoa, cfl=0.5, maxdt=1.0, K_p=0.0, K_u=0.75):
+    """(free-stream tuple for the library, FP32 parameter vector for the
+    oracle): the same double-precision host values rounded to FP32."""
+    fs = ausm.freestream(mach, aoa)
+    par = make_params(mach, aoa, cfl=cfl, maxdt=maxdt, K_p=K_p, K_u=K_u)
+    return fs, par
+
+
+def new_pair(vtx, mach, aoa, scheme, limiter, math_mode="exact", refresh=1, tile=0, cfl=0.5):
+    fs, par = flow(mach, aoa, cfl=cfl)
+    N1, M, _ = vtx.shape
+    s = ausm.Solver(M, N1 - 1)
+    s.set_vertices(vtx)
+    configure(s, fs, par, scheme, limiter, math_mode, refresh, tile)
+    ref = Solver(32, par, scheme, limiter, vtx, refresh=refresh)
+    return s, ref, par
+
+
+def max_rel(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    d = np.abs(a - b).reshape(-1, 4).max(axis=0)
+    sc = np.abs(b).reshape(-1, 4).max(axis=0)
+    return float((d / np.maximum(sc, 1e-30)).max())
+
+
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("limiter", LIMITERS)
+def test_residual_exact_bitwise(scheme, limiter):
+    vtx, _, _ = make_grid(96, 40, code=2412, iters=150)
+    s, ref, par = new_pair(vtx, 0.75, 2.0, scheme, limiter)
+    st = perturbed_state(par, 96, 40, seed=scheme * 7 + limiter)
+    s.set_state(st)
+    r_gpu = s.residual()
+    r_ref = Oracle(32).residual(par, scheme, limiter, vtx, st)
+    assert np.array_equal(r_gpu, r_ref), max_rel(r_gpu, r_ref)
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("mach,aoa", [(0.8, 1.25), (2.0, 0.0)])
+def test_steps_exact_bitwise(scheme, mach, aoa):
+    M, N = 96, 40
+    vtx, _, _ = make_grid(M, N, code=12, iters=150)
+    s, ref, par = new_pair(vtx, mach, aoa, scheme, VANALBADA)
+    st = perturbed_state(par, M, N, seed=11 + scheme)
+    s.set_state(st)
+    ref.set_state(st)
+    for chunk in (1, 5, 40):  # 40 steps crosses one captured 32-step graph
+        info = s.step(chunk)
+        ref.step(chunk)
+        got = s.state()
+        assert np.array_equal(got[1:N + 1], ref.state[1:N + 1]), (chunk, max_rel(got, ref.state))
+        assert info["dt"] == ref.last_dt
+    s.close()
+
+
+def test_dt_refresh_cadence_bitwise():
+    """steps_per_dt_refresh = 7 (the reference refreshes dt once per frame)."""
+    M, N = 70, 24
+    vtx, _, _ = make_grid(M, N, code=4412, iters=100)
+    s, ref, par = new_pair(vtx, 0.6, 3.0, SLAU2, VANALBADA, refresh=7)
+    st = perturbed_state(par, M, N, seed=5)
+    s.set_state(st)
+    ref.set_state(st)
+    for chunk in (3, 11, 70):
+        s.step(chunk)
+        ref.step(chunk)
+        assert np.array_equal(s.state()[1:N + 1], ref.state[1:N + 1]), chunk
+    s.close()
+
+
+def test_rest_start_bitwise():
+    """The reference's own start: air at rest, free stream switched on."""
+    M, N = 64, 20
+    vtx, _, _ = make_grid(M, N, code=12, iters=80, rect=True)
+    fs, par = flow(0.5, 0.0, cfl=1.5)
+    s = ausm.Solver(M, N)
+    s.set_vertices(vtx)
+    configure(s, fs, par, SLAU2, VANALBADA)
+    s.prepare_state(freestream=False)
+    ref = Solver(32, par, SLAU2, VANALBADA, vtx, freestream=False)
+    s.step(25)
+    ref.step(25)
+    assert np.array_equal(s.state()[1:N + 1], ref.state[1:N + 1])
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_shock_state_exact_bitwise(scheme):
+    """A shock-containing state from a supersonic run, then exact steps."""
+    M, N = 128, 48
+    vtx, _, _ = make_grid(M, N, code=12, iters=200, radius=6.0)
+    s, ref, par = new_pair(vtx, 2.0, 2.0, scheme, VANALBADA, cfl=0.5)
+    s.prepare_state(freestream=True)
+    s.step(300, info=False)
+    shock = s.state()
+    rho = shock[1:N + 1, :, 0]
+    assert rho.max() / rho.min() > 2.0  # bow shock present
+    s.set_state(shock)
+    ref.set_state(shock)
+    s.step(3)
+    ref.step(3)
+    assert np.array_equal(s.state()[1:N + 1], ref.state[1:N + 1])
+    assert np.array_equal(s.residual(), Oracle(32).residual(par, scheme, VANALBADA, vtx, s.state()))
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_fast_math_within_tolerance(scheme):
+    """Fast math: per-step residual max-rel <= 1e-5 against ref32."""
+    M, N = 128, 48
+    vtx, _, _ = make_grid(M, N, code=2412, iters=150)
+    s, ref, par = new_pair(vtx, 0.8, 1.25, scheme, VANALBADA, math_mode="fast")
+    st = perturbed_state(par, M, N, seed=3)
+    s.set_state(st)
+    r = s.residual()
+    r_ref = Oracle(32).residual(par, scheme, VANALBADA, vtx, st)
+    assert max_rel(r, r_ref) <= FAST_TOL, max_rel(r, r_ref)
+    # the per-step residual along a trajectory stays within tolerance
+    ref.set_state(st)
+    for _ in range(3):
+        s.step(10)
+        ref.step(10)
+        cur = s.state()
+        rr = Oracle(32).residual(par, scheme, VANALBADA, vtx, cur)
+        assert max_rel(s.residual(), rr) <= FAST_TOL
+        assert max_rel(cur[1:N + 1], ref.state[1:N + 1]) < 1e-4
+    s.close()
+
+
+@pytest.mark.parametrize("limiter", [MINMOD, VANLEER, NONE])
+def test_fast_math_limiters(limiter):
+    M, N = 96, 32
+    vtx, _, _ = make_grid(M, N, code=12, iters=100)
+    s, ref, par = new_pair(vtx, 1.4, 1.0, SLAU2, limiter, math_mode="fast")
+    st = perturbed_state(par, M, N, seed=9)
+    s.set_state(st)
+    r_ref = Oracle(32).residual(par, SLAU2, limiter, vtx, st)
+    assert max_rel(s.residual(), r_ref) <= FAST_TOL
+    s.close()
+
+
+def test_grid_fp64_bitwise():
+    M, N = 80, 30
+    m, p, t = naca4_digits(2412)
+    body = naca4_boundary(M, m, p, t, size=0.5, expo=1.4, spacing=0)
+    outer = circle_boundary(M, 8.0)
+    s = ausm.Solver(M, N)
+    s.update_grid_boundaries(body, outer)
+    s.prepare_grid(iters=301)
+    v64 = s.vertices(fp64=True)
+    ref = Oracle(64).grid(body, outer, N, 301)
+    assert np.array_equal(v64, ref)
+    assert np.array_equal(s.vertices(), ref.astype(np.float32))
+    # metrics from the FP32 vertices, as the reference's finalize pass
+    area, flen = s.metrics()
+    ra, rf = Oracle(32).finalize(ref.astype(np.float32))
+    assert np.array_equal(area, ra) and np.array_equal(flen, rf)
+    s.close()
+
+
+def test_grid_sor_converges_to_jacobi_fixed_point():
+    M, N = 64, 16
+    body = circle_boundary(M, 1.0)
+    outer = circle_boundary(M, 3.0)
+    s = ausm.Solver(M, N)
+    s.update_grid_boundaries(body, outer)
+    s.prepare_grid(iters=1500, method="sor", omega=1.5)
+    v = s.vertices(fp64=True)
+    r = np.hypot(v[..., 0], v[..., 1]).mean(axis=1)
+    ref = Oracle(64).grid(body, outer, N, 8000)
+    rr = np.hypot(ref[..., 0], ref[..., 1]).mean(axis=1)
+    assert np.abs(r - rr).max() < 1e-6
+    s.close()
+
+
+def test_wave_speeds_forces_bitwise():
+    M, N = 96, 40
+    vtx, _, _ = make_grid(M, N, code=2412, iters=150)
+    s, ref, par = new_pair(vtx, 0.7, 4.0, SLAU2, VANALBADA)
+    st = perturbed_state(par, M, N, seed=21, amp=0.05)
+    s.set_state(st)
+    o = Oracle(32)
+    area, flen = o.finalize(vtx)
+    ws, mx = s.wave_speeds()
+    mref, wref = o.max_ws(par, st, area, flen, want_field=True)
+    assert np.array_equal(ws, wref) and mx == mref
+    f = s.forces(ref_length=1.0, want_cp=True)
+    fo, cpo = o.forces(par, st, vtx, ref_len=1.0, want_cp=True)
+    assert [f["fx"], f["fy"], f["cl"], f["cd"]] == list(fo)
+    assert np.array_equal(f["cp"], cpo)
+    s.close()
+
+
+def test_state_roundtrip_and_ghost_rows():
+    M, N = 90, 33
+    vtx, _, _ = make_grid(M, N, code=12, iters=100)
+    for mach in (0.5, 2.5):  # subsonic Riemann and supersonic far-field branches
+        s, ref, par = new_pair(vtx, mach, 10.0, SLAU2, VANALBADA)
+        st = perturbed_state(par, M, N, seed=2, amp=0.05)
+        s.set_state(st)
+        got = s.state()
+        assert np.array_equal(got[1:N + 1], st[1:N + 1])
+        b = Oracle(32).boundary(par, vtx, st)
+        assert np.array_equal(got[0], b[0]) and np.array_equal(got[N + 1:], b[N + 1:])
+        s.close()
+
+
+def test_tiles_graphs_and_reruns_identical():
+    M, N = 150, 37
+    vtx, _, _ = make_grid(M, N, code=4412, iters=100)
+    outs = []
+    for tile in (1, 2, 4, 8):
+        s, _, par = new_pair(vtx, 0.85, 1.0, SLAU2, VANALBADA, tile=tile)
+        s.set_state(perturbed_state(par, M, N, seed=4))
+        s.step(70)
+        outs.append(s.state())
+        s.close()
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_small_grid_wraps_within_a_wavefront():
+    M, N = 20, 6
+    vtx, _, _ = make_grid(M, N, code=12, iters=40, radius=4.0)
+    s, ref, par = new_pair(vtx, 0.6, 5.0, AUSMUP, VANALBADA)
+    st = perturbed_state(par, M, N, seed=8)
+    s.set_state(st)
+    ref.set_state(st)
+    s.step(12)
+    ref.step(12)
+    assert np.array_equal(s.state()[1:N + 1], ref.state[1:N + 1])
+    s.close()
+
+
+def test_residual_norm_fp64():
+    M, N = 96, 40
+    vtx, _, _ = make_grid(M, N, code=2412, iters=150)
+    s, ref, par = new_pair(vtx, 0.75, 2.0, SLAU2, VANALBADA)
+    st = perturbed_state(par, M, N, seed=17)
+    s.set_state(st)
+    l2 = s.residual_norm()
+    r = Oracle(32).residual(par, SLAU2, VANALBADA, vtx, st)
+    expect = math.sqrt(float((r[..., 0].astype(np.float64) ** 2).sum()))
+    assert abs(l2 - expect) <= 1e-12 * expect
+    info = s.step(1)
+    assert abs(info["res_l2"] - expect) <= 1e-12 * expect
+    s.close()
+
+
+def test_errors_are_reported():
+    s = ausm.Solver(32, 8)
+    with pytest.raises(ausm.AusmError) as e:
+        s.step(1)
+    assert e.value.code == -4
+    with pytest.raises(ausm.AusmError):
+        s.set_scheme(7, 0)
+    with pytest.raises(ausm.AusmError):
+        s.set_cfl(-1.0)
+    s.close()
+
+
+def test_nonfinite_is_flagged():
+    M, N = 64, 16
+    vtx, _, _ = make_grid(M, N, code=12, iters=50)
+    s, _, par = new_pair(vtx, 0.5, 0.0, SLAU2, VANALBADA)
+    st = perturbed_state(par, M, N, seed=1)
+    st[5, 10, 3] = -5.0  # negative energy -> negative pressure -> NaN sound speed
+    s.set_state(st)
+    with pytest.raises(ausm.AusmError) as e:
+        s.step(2)
+    assert e.value.code == -5
+    s.close()
