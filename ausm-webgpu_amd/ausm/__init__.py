@@ -2,10 +2,11 @@
 MUSCL, TVD-RK3) behind the C ABI of include/ausm.h."""
 from .lib import (AUSMUP, GRID_JACOBI, GRID_SOR, INIT_FREESTREAM, INIT_REST, MATH_EXACT,
                   MATH_FAST, MINMOD, NOLIMITER, SLAU, SLAU2, VANALBADA, VANLEER, AusmError, load)
+from .batch import Batch
 from .solver import (Solver, circle_boundary, freestream, naca4_boundary, naca4_digits,
                      rect_boundary)
 
-__all__ = ["Solver", "AusmError", "load", "naca4_boundary", "naca4_digits", "circle_boundary",
+__all__ = ["Solver", "Batch", "AusmError", "load", "naca4_boundary", "naca4_digits", "circle_boundary",
            "rect_boundary", "freestream", "SLAU2", "SLAU", "AUSMUP", "VANALBADA", "MINMOD",
            "VANLEER", "NOLIMITER", "MATH_EXACT", "MATH_FAST", "INIT_REST", "INIT_FREESTREAM",
            "GRID_JACOBI", "GRID_SOR"]

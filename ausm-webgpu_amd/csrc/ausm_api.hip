@@ -208,7 +208,7 @@ int ausm_ctx_create(int device, int M, int N, ausm_ctx** out) {
   ok = ok && hipMalloc(&c->vy, sizeof(float) * nv) == hipSuccess;
   for (int k = 0; k < 4 && ok; ++k) ok = hipMalloc(&c->g64[k], sizeof(double) * nv) == hipSuccess;
   ok = ok && hipMalloc(&c->ws_part, sizeof(float) * maxb) == hipSuccess;
-  ok = ok && hipMalloc(&c->res_part, sizeof(double) * maxb) == hipSuccess;
+  ok = ok && hipMalloc(&c->res_part, sizeof(double) * (size_t)N * stage_tiles_x(M)) == hipSuccess;
   ok = ok && hipMalloc(&c->scal_f, sizeof(float) * 8) == hipSuccess;
   ok = ok && hipMalloc(&c->scal_d, sizeof(double) * 8) == hipSuccess;
   ok = ok && hipMalloc(&c->dpar, sizeof(DevParams)) == hipSuccess;
@@ -642,7 +642,7 @@ int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
   if (info) {
     // density-residual norm of the last step's stage 1 and the current max
     // wave speed; one small read-back
-    CHECK_HIP(launch_reduce_sum_f64(c->res_part, stage_blocks(c->M, c->N, ty), c->scal_d, c->stream), "residual norm");
+    CHECK_HIP(launch_reduce_sum_f64(c->res_part, c->N * stage_tiles_x(c->M), c->scal_d, c->stream), "residual norm");
     CHECK_HIP(launch_reduce_max(c->ws_part, c->n_ws_valid, c->scal_f + 1, c->stream), "max wave speed");
     CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "info");
     CHECK_HIP(hipMemcpyAsync((char*)c->pinned + 8, c->scal_f, 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "info");
@@ -719,7 +719,7 @@ int ausm_residual_norm(ausm_ctx* c, double* l2) {
   a.qout = c->q[1];
   a.fresh = 0;
   CHECK_HIP(launch_stage(a, c->scheme, c->math, ty, c->stream), "residual norm");
-  CHECK_HIP(launch_reduce_sum_f64(c->res_part, stage_blocks(c->M, c->N, ty), c->scal_d, c->stream), "residual norm");
+  CHECK_HIP(launch_reduce_sum_f64(c->res_part, c->N * stage_tiles_x(c->M), c->scal_d, c->stream), "residual norm");
   CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "residual norm");
   CHECK_HIP(hipStreamSynchronize(c->stream), "residual norm");
   *l2 = *(double*)c->pinned;

@@ -179,13 +179,19 @@ __global__ void reduce_max_kernel(const float* part, int n, float* out) {
   if (threadIdx.x == 0) *out = sm[0];
 }
 
-// sum of FP64 partials in index order (one thread: deterministic and the
-// partial counts are small)
+// square root of the sum of FP64 partials, in a fixed order (strided
+// per-thread sums, then a fixed tree): deterministic for a given n
 __global__ void reduce_sum_f64_kernel(const double* part, int n, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double sm[256];
   double s = 0.0;
-  for (int i = 0; i < n; ++i) s += part[i];
-  *out = sqrt(s);
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sm[threadIdx.x] += sm[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(sm[0]);
 }
 
 // --------------------------------------------------------------------------
@@ -349,7 +355,7 @@ hipError_t launch_reduce_max(const float* part, int n, float* out, hipStream_t s
   return hipGetLastError();
 }
 hipError_t launch_reduce_sum_f64(const double* part, int n, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_sum_f64_kernel, dim3(1), dim3(64), 0, s, part, n, out);
+  hipLaunchKernelGGL(reduce_sum_f64_kernel, dim3(1), dim3(256), 0, s, part, n, out);
   return hipGetLastError();
 }
 hipError_t launch_forces(const float* q, const float* vx, const float* vy, const DevParams* par,

@@ -1,0 +1,179 @@
+// Batched solves: one persistent fused RK-stage kernel walks a tile list
+// that spans every grid in the batch.
+//
+//  * The solve table holds each solve's buffers, sizes, scheme, limiter and
+//    free-stream parameters; the item list holds (solve, workgroup tile)
+//    pairs sorted by scheme and solve, so a workgroup's run of items mostly
+//    uses one scheme's code.
+//  * The launch is persistent: a fixed number of workgroups per CU, each
+//    taking one contiguous run of items (runs are dealt so that each XCD
+//    gets neighbouring tiles and keeps their halos in its own L2).
+//  * Per-solve dt comes from a small kernel that reduces each solve's
+//    wave-speed maxima once per step (max is exact, so dt is the same
+//    whatever else is in the batch); per-solve residual norms are reduced
+//    from (row, strip) partials in a fixed order, so a solve's results are
+//    bitwise the same alone and in any mix.
+//  * Item count and solve flags live in device memory, so one captured
+//    graph serves every batch composition.
+#include "batch_kernels.hpp"
+#include "stage_tile.hpp"
+
+namespace ausm {
+
+__device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, int refresh) {
+  StageArgs a{};
+  a.vx = S.vx;
+  a.vy = S.vy;
+  a.par = nullptr;
+  a.ws_in = S.ws_part;
+  a.ws_out = S.ws_part;
+  a.res_out = S.res_part;
+  a.maxws_cur = S.maxws_cur;
+  a.n_ws_in = S.n_ws;
+  a.M = S.M;
+  a.N = S.N;
+  a.P = S.P;
+  a.PS = S.PS;
+  a.limiter = S.limiter;
+  a.tiles_x = S.tiles_x;
+  a.fresh = 0;
+  a.refresh = refresh;
+  if (stage == 1) {
+    a.stage = 1;
+    a.qin = S.q[0];
+    a.qn = S.q[0];
+    a.qout = S.q[1];
+  } else if (stage == 2) {
+    a.stage = 2;
+    a.qin = S.q[1];
+    a.qn = S.q[0];
+    a.qout = S.q[2];
+  } else if (stage == 3) {
+    a.stage = 3;
+    a.qin = S.q[2];
+    a.qn = S.q[0];
+    a.qout = S.q[0];
+  } else {
+    // schedule tuning: stage-2 arithmetic from Q^n into the scratch Q2
+    a.stage = 2;
+    a.qin = S.q[0];
+    a.qn = S.q[0];
+    a.qout = S.q[2];
+    a.refresh = 0;
+  }
+  return a;
+}
+
+template <bool FAST, int TY>
+__global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
+  const int n = *b.n_items;
+  const int g = xcd_remap(blockIdx.x, gridDim.x);
+  const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int lo = g * per;
+  const int hi = min(n, lo + per);
+  for (int it = lo; it < hi; ++it) {
+    const BatchItem item = b.items[it];
+    const SolveDev* S = b.solves + item.solve;
+    const StageArgs a = batch_args(*S, b.stage, b.refresh);
+    const DevParams P = S->par;
+    const int tx = item.tile % S->tiles_x;
+    const int tyb = item.tile / S->tiles_x;
+    const float dt = fminf(P.maxdt, P.cfl / *S->maxws_cur);
+    float ws;
+    switch (S->scheme) {
+      case SCHEME_SLAU2: ws = stage_tile<SCHEME_SLAU2, FAST, TY>(P, a, tx, tyb, dt); break;
+      case SCHEME_SLAU: ws = stage_tile<SCHEME_SLAU, FAST, TY>(P, a, tx, tyb, dt); break;
+      default: ws = stage_tile<SCHEME_AUSMUP, FAST, TY>(P, a, tx, tyb, dt); break;
+    }
+    if (a.stage == 3 && a.refresh) block_ws_partial(ws, S->ws_part + item.tile);
+  }
+}
+
+// per-solve max wave speed of the last refresh -> maxws_cur (one workgroup
+// per table slot; inactive slots are skipped)
+__global__ void batch_dt_kernel(const SolveDev* solves, const int* n_solves) {
+  const int s = blockIdx.x;
+  if (s >= *n_solves) return;
+  const SolveDev& S = solves[s];
+  if (!S.active) return;
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < S.n_ws; i += 64) m = nan_max(m, S.ws_part[i]);
+  m = wave_max(m);
+  if (threadIdx.x == 0) *S.maxws_cur = m;
+}
+
+// per-solve FP64 residual norm (fixed-order reduction of the (row, strip)
+// partials) into out[2*s], and into out[2*s+1] the larger of the max wave
+// speed of the last step and that of the state it produced (NaN-propagating,
+// so a solution that turned non-finite in the last step is caught now)
+__global__ void batch_norm_kernel(const SolveDev* solves, const int* n_solves, double* out) {
+  const int s = blockIdx.x;
+  if (s >= *n_solves) return;
+  const SolveDev& S = solves[s];
+  if (!S.active) return;
+  __shared__ double sm[256];
+  __shared__ float smw[256];
+  const int n = S.N * S.tiles_x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += S.res_part[i];
+  float w = 0.0f;
+  for (int i = threadIdx.x; i < S.n_ws; i += 256) w = nan_max(w, S.ws_part[i]);
+  sm[threadIdx.x] = acc;
+  smw[threadIdx.x] = w;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      sm[threadIdx.x] += sm[threadIdx.x + k];
+      smw[threadIdx.x] = nan_max(smw[threadIdx.x], smw[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * s] = sqrt(sm[0]);
+    out[2 * s + 1] = (double)nan_max(*S.maxws_cur, smw[0]);
+  }
+}
+
+// a solve's initial wave-speed partials in the stage-3 layout: the max of
+// the per-block values in entry 0, zeros after it
+__global__ void batch_seed_ws_kernel(const float* part, int n_part, float* ws_part, int n_ws) {
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < n_part; i += 64) m = nan_max(m, part[i]);
+  m = wave_max(m);
+  for (int i = threadIdx.x; i < n_ws; i += 64) ws_part[i] = (i == 0) ? m : 0.0f;
+}
+
+template <bool FAST>
+static hipError_t launch_bt(const BatchArgs& b, int ty, int blocks, hipStream_t s) {
+  switch (ty) {
+    case 1: hipLaunchKernelGGL((batch_stage_kernel<FAST, 1>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 2: hipLaunchKernelGGL((batch_stage_kernel<FAST, 2>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 4: hipLaunchKernelGGL((batch_stage_kernel<FAST, 4>), dim3(blocks), dim3(256), 0, s, b); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_stage(const BatchArgs& b, int fast, int ty, int blocks, hipStream_t s) {
+  return fast ? launch_bt<true>(b, ty, blocks, s) : launch_bt<false>(b, ty, blocks, s);
+}
+
+hipError_t launch_batch_dt(const SolveDev* solves, const int* n_solves, int max_solves,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(batch_dt_kernel, dim3(max_solves), dim3(64), 0, s, solves, n_solves);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_norm(const SolveDev* solves, const int* n_solves, int max_solves,
+                             double* out, hipStream_t s) {
+  hipLaunchKernelGGL(batch_norm_kernel, dim3(max_solves), dim3(256), 0, s, solves, n_solves, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_seed_ws(const float* part, int n_part, float* ws_part, int n_ws,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(batch_seed_ws_kernel, dim3(1), dim3(64), 0, s, part, n_part, ws_part, n_ws);
+  return hipGetLastError();
+}
+
+}  // namespace ausm

@@ -30,7 +30,7 @@ struct ausm_ctx {
   double* g64[4] = {nullptr, nullptr, nullptr, nullptr};  // FP64 grid ping-pong (x,y,x',y')
   int g_cur = -1;                                          // 0 or 2: which pair holds the grid
   float* ws_part = nullptr;   // per-workgroup max wave speeds
-  double* res_part = nullptr; // per-workgroup density-residual sums of squares
+  double* res_part = nullptr; // density-residual sums of squares per (row, 60-column strip)
   float* scal_f = nullptr;    // [0] max wave speed of the current dt, [1..] scratch
   double* scal_d = nullptr;   // [0] residual norm, [2..5] forces
   float* stage_buf = nullptr; // AoS staging, (N+3) x M x 4 floats

@@ -22,7 +22,7 @@ struct StageArgs {
   const DevParams* par;
   const float* ws_in;  // per-block max wave speeds of the last refresh
   float* ws_out;       // per-block max wave speeds of this stage's output (stage 3)
-  double* res_out;     // per-block sums of squares of the density residual (stage 1)
+  double* res_out;     // sums of squares of the density residual per (row, strip), N x tiles_x (stage 1)
   float* maxws_cur;    // max wave speed the current step's dt is based on
   int n_ws_in;
   int M, N, P, PS;

@@ -187,12 +187,16 @@ int ausm_batch_create(int device, int max_solves, size_t hbm_budget_bytes, ausm_
 int ausm_batch_destroy(ausm_batch* b);
 int ausm_batch_stream(ausm_batch* b, void** stream);
 int ausm_batch_set_math(ausm_batch* b, int math);
+/* fix the persistent schedule: rows per wavefront tile (1, 2, 4) and
+ * workgroups per CU (1..16); 0 leaves a value to the per-mix autotuner */
+int ausm_batch_set_schedule(ausm_batch* b, int rows_per_wave, int blocks_per_cu);
 /* queue a solve; it is admitted when a slot and memory are free */
 int ausm_batch_submit(ausm_batch* b, const ausm_solve_desc* desc, uint64_t* id);
-/* advance every running solve by `nsteps` steps (one persistent launch per
- * RK stage per step), check convergence every `check_every` steps, retire
- * finished solves and admit queued ones; returns the number still active or
- * queued through *pending. */
+/* advance every running solve by `nsteps` steps, rounded up to a multiple
+ * of `check_every` (one persistent launch per RK stage per step); after every
+ * `check_every` steps convergence is checked, finished solves retire and
+ * queued ones are admitted.  *pending gets the number running or queued.
+ * Step caps take effect at the first check at or after max_steps. */
 int ausm_batch_advance(ausm_batch* b, int nsteps, int check_every, int* pending);
 int ausm_batch_poll(ausm_batch* b, uint64_t id, ausm_solve_status* st);
 /* copy a solve's state (finished or running) in the reference layout */
