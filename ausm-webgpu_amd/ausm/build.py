@@ -52,7 +52,33 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
+    build_node_addon(verbose)
     return LIB
+
+
+NODE_DIR = os.path.join(PKG, "node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+ADDON = os.path.join(NODE_DIR, "ausm_napi.node")
+
+
+def build_node_addon(verbose=True):
+    """Node-API addon over libausm (plain C, linked against libausm.so with
+    an $ORIGIN rpath); skipped when the node headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print("node headers not found: Node-API addon not built", file=sys.stderr)
+        return None
+    cc = os.environ.get("CC", "gcc")
+    cmd = [cc, "-shared", "-fPIC", "-O2", "-Wall", "-I" + NODE_INCLUDE, "-I" + INCLUDE,
+           "-DNODE_GYP_MODULE_NAME=ausm_napi", os.path.join(NODE_DIR, "ausm_napi.c"),
+           "-o", ADDON + ".tmp", "-L" + HERE, "-lausm", "-Wl,-rpath,$ORIGIN/../ausm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Node-API addon build failed:\n{r.stderr[-4000:]}")
+    os.replace(ADDON + ".tmp", ADDON)
+    if verbose:
+        print(f"built {ADDON}", file=sys.stderr)
+    return ADDON
 
 
 if __name__ == "__main__":

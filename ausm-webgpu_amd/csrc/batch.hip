@@ -358,12 +358,16 @@ static float time_schedule(ausm_batch* b, int ty, int bpc) {
   (void)hipEventCreate(&e1);
   const int blocks = b->cus * bpc;
   for (int w = 0; w < 2; ++w) (void)launch_batch_stage(a, b->math, ty, blocks, b->stream);
-  (void)hipEventRecord(e0, b->stream);
-  for (int k = 0; k < 3; ++k) (void)launch_batch_stage(a, b->math, ty, blocks, b->stream);
-  (void)hipEventRecord(e1, b->stream);
-  (void)hipEventSynchronize(e1);
   float ms = 1e30f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
+  for (int round = 0; round < 2; ++round) {
+    (void)hipEventRecord(e0, b->stream);
+    for (int k = 0; k < 3; ++k) (void)launch_batch_stage(a, b->math, ty, blocks, b->stream);
+    (void)hipEventRecord(e1, b->stream);
+    (void)hipEventSynchronize(e1);
+    float t = 1e30f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    ms = std::min(ms, t);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return ms;

@@ -43,6 +43,7 @@ struct ausm_ctx {
   int scheme = 0, limiter = 0, math = 0, refresh = 1, tile = 0;
   bool use_graphs = true;
   long long steps = 0;
+  int since_refresh = 0;  // steps since the wave speed was last refreshed
   std::vector<ausm::GraphEntry> graphs;
 };
 

@@ -1,0 +1,139 @@
+'use strict';
+// Node.js host for the MI355X solver step, keeping the reference's host
+// surface (huj31415/ausm-webgpu src/main.js, src/grid.js, src/util.js):
+// updateGridBoundaries -> prepareGrid -> prepareState -> frame loop, with
+// updateSolver to swap the flux scheme.  The compute runs in libausm through
+// the Node-API addon (ausm_napi.node); nothing here does arithmetic on the
+// solution.
+const path = require('path');
+
+const native = require(path.join(__dirname, 'ausm_napi.node'));
+
+// flux scheme table, in the reference order (src/util.js:114-118)
+const solvers = Object.freeze({ 'SLAU2': 0, 'SLAU': 1, 'AUSM+-up': 2 });
+const limiters = Object.freeze({ vanAlbada: 0, minmod: 1, vanLeer: 2, none: 3 });
+const INIT_REST = 0;
+const INIT_FREESTREAM = 1;
+
+function toF64Pairs(coords) {
+  if (coords instanceof Float64Array) return coords;
+  const out = new Float64Array(coords.length * 2);
+  coords.forEach((p, i) => { out[2 * i] = p[0]; out[2 * i + 1] = p[1]; });
+  return out;
+}
+
+// generateNACA4Boundary (src/grid.js:38-52) for all M points at once
+function naca4Boundary(M, naca4, size = 1.0, xOffset = 0, pointDistExponent = 7 / 5, spacing = 0) {
+  const t = naca4 % 100 / 100;
+  const m = Math.floor(naca4 / 1000) / 100;
+  const p = Math.floor(naca4 / 100) % 10 / 10;
+  return native.boundaryNaca4(m, p, t, size, xOffset, pointDistExponent, spacing, M);
+}
+const circularOuterBoundary = (M, radius = 1) => native.boundaryCircle(radius, M);
+const rectangularOuterBoundary = (M, lengthRatio = 1.5) => native.boundaryRect(lengthRatio, M);
+
+class AusmSolver {
+  // simulationDomain = [M, N] (src/grid.js:12)
+  constructor({ M, N, device = 0 }) {
+    this.M = M;
+    this.N = N;
+    this.ctx = native.createContext(M, N, device);
+    // physics and solver defaults (src/util.js:76-99, src/main.js:874-887)
+    this.gamma = 1.4;
+    this.inPressure = 1.0 / this.gamma;
+    this.inRho = 1.0;
+    this.maxdt = 1e-4;
+    this.cflFactor = 1.5;
+    this.K_p = 0;
+    this.K_u = 0.75;
+    this.muscl = true;
+    this.limiter = 'vanAlbada';
+    this.inflowVel = 3.8;
+    this.actualInflowVel = 0;
+    this.velRampUpStrength = 128;
+    this.xyAoA = [1, 0];
+    this.stepsPerFrame = 70;
+    this.solver = 'SLAU2';
+    this.mathMode = 0;
+    this._body = null;
+    this._outer = null;
+  }
+
+  setAoA(deg) {
+    const r = deg * Math.PI / 180;
+    this.xyAoA = [Math.cos(r), Math.sin(r)];
+  }
+
+  // updateGridBoundaries (src/grid.js:267-296): body row and far-field row
+  updateGridBoundaries(objCoords, boundCoords) {
+    this._body = toF64Pairs(objCoords);
+    this._outer = toF64Pairs(boundCoords);
+  }
+
+  // prepareGrid (src/main.js:658-707): interpolation, elliptic smoothing and
+  // metrics on the GPU (FP64); the reference runs 1e4 Jacobi sweeps
+  prepareGrid(iterations = 1e4) {
+    native.gridFromBoundary(this.ctx, this._body, this._outer, iterations, 0, 1.0);
+  }
+
+  _uploadUniforms() {
+    const v = this.actualInflowVel;
+    native.setFreestream(this.ctx, this.inRho, this.inPressure, v * this.xyAoA[0], v * this.xyAoA[1],
+                         this.gamma);
+    native.setScheme(this.ctx, solvers[this.solver], this.muscl ? limiters[this.limiter] : limiters.none,
+                     this.K_p, this.K_u);
+    native.setCfl(this.ctx, this.cflFactor, this.maxdt, this.stepsPerFrame);
+    native.setMath(this.ctx, this.mathMode);
+  }
+
+  // prepareState (src/main.js:631-655): air at rest, ramp restarted
+  prepareState(mode = INIT_REST) {
+    this._uploadUniforms();
+    native.initState(this.ctx, mode);
+    this.actualInflowVel = 0;
+  }
+
+  // updateSolver (src/main.js:629): swap the flux scheme, keep the state
+  updateSolver(name) {
+    if (!(name in solvers)) throw new Error(`unknown solver ${name}`);
+    this.solver = name;
+    native.setScheme(this.ctx, solvers[name], this.muscl ? limiters[this.limiter] : limiters.none,
+                     this.K_p, this.K_u);
+  }
+
+  // One frame of render() (src/main.js:735-784): upload the uniforms, take
+  // stepsPerFrame steps with one dt for the frame, then ramp the inflow for
+  // the next frame.
+  frame() {
+    this._uploadUniforms();
+    const info = native.step(this.ctx, this.stepsPerFrame);
+    this._ramp();
+    return info;
+  }
+
+  async frameAsync() {
+    this._uploadUniforms();
+    const info = await native.stepAsync(this.ctx, this.stepsPerFrame);
+    this._ramp();
+    return info;
+  }
+
+  _ramp() {
+    this.actualInflowVel += (this.inflowVel - this.actualInflowVel) /
+                            (this.velRampUpStrength / this.stepsPerFrame * 50);
+  }
+
+  step(n) { return native.step(this.ctx, n); }
+  stepAsync(n) { return native.stepAsync(this.ctx, n); }
+  state() { return native.getState(this.ctx, this.M, this.N); }
+  setState(q) { native.setState(this.ctx, q); }
+  vertices() { return native.getVertices(this.ctx, this.M, this.N); }
+  setVertices(v) { native.setVertices(this.ctx, v); }
+  residual() { return native.residual(this.ctx, this.M, this.N); }
+  forces(refLength = 1.0) { return native.forces(this.ctx, refLength, this.M); }
+}
+
+module.exports = {
+  AusmSolver, solvers, limiters, naca4Boundary, circularOuterBoundary, rectangularOuterBoundary,
+  INIT_REST, INIT_FREESTREAM, native,
+};

@@ -1,0 +1,401 @@
+/*
+ * Node-API binding of libausm (include/ausm.h) for Node.js hosts.
+ *
+ * Exposes the C ABI one-to-one to JavaScript: a context is an external
+ * value; arrays cross as typed arrays (Float32Array / Float64Array) that are
+ * copied by the library; errors become JS exceptions carrying
+ * ausm_last_error().  stepAsync() runs the steps on the libuv thread pool
+ * (napi_async_work) and resolves a Promise, so long step loops do not block
+ * the event loop.
+ */
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ausm.h"
+
+#define NAPI_CALL(env, call)                                  \
+  do {                                                        \
+    if ((call) != napi_ok) {                                  \
+      napi_throw_error((env), NULL, "N-API call failed: " #call); \
+      return NULL;                                            \
+    }                                                         \
+  } while (0)
+
+static napi_value throw_ausm(napi_env env, int rc) {
+  char msg[600];
+  snprintf(msg, sizeof msg, "ausm error %d: %s", rc, ausm_last_error());
+  napi_throw_error(env, NULL, msg);
+  return NULL;
+}
+
+#define AUSM_CALL(env, expr)               \
+  do {                                     \
+    int _rc = (expr);                      \
+    if (_rc != AUSM_OK) return throw_ausm((env), _rc); \
+  } while (0)
+
+static void ctx_finalize(napi_env env, void* data, void* hint) {
+  (void)env;
+  (void)hint;
+  if (data) ausm_ctx_destroy((ausm_ctx*)data);
+}
+
+/* ---- argument helpers ---------------------------------------------------- */
+static int get_args(napi_env env, napi_callback_info info, size_t n, napi_value* argv) {
+  size_t argc = n;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok) return 0;
+  if (argc < n) {
+    napi_throw_type_error(env, NULL, "too few arguments");
+    return 0;
+  }
+  return 1;
+}
+
+static double num(napi_env env, napi_value v) {
+  double d = 0.0;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+static ausm_ctx* ctx_of(napi_env env, napi_value v) {
+  void* p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "expected an ausm context");
+    return NULL;
+  }
+  return (ausm_ctx*)p;
+}
+
+/* pointer and element count of a typed array of the given type */
+static void* typed(napi_env env, napi_value v, napi_typedarray_type want, size_t* n) {
+  napi_typedarray_type t;
+  size_t len = 0;
+  void* data = NULL;
+  napi_value ab;
+  size_t off = 0;
+  if (napi_get_typedarray_info(env, v, &t, &len, &data, &ab, &off) != napi_ok || t != want) {
+    napi_throw_type_error(env, NULL,
+                          want == napi_float32_array ? "expected a Float32Array" : "expected a Float64Array");
+    return NULL;
+  }
+  *n = len;
+  return data;
+}
+
+static napi_value new_typed(napi_env env, napi_typedarray_type t, size_t n, void** data) {
+  napi_value ab, arr;
+  const size_t esz = t == napi_float64_array ? 8 : 4;
+  if (napi_create_arraybuffer(env, n * esz, data, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, t, n, ab, 0, &arr) != napi_ok) return NULL;
+  return arr;
+}
+
+static void set_num(napi_env env, napi_value obj, const char* k, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, obj, k, x);
+}
+
+/* ---- context --------------------------------------------------------------- */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  ausm_ctx* c = NULL;
+  AUSM_CALL(env, ausm_ctx_create((int)num(env, a[2]), (int)num(env, a[0]), (int)num(env, a[1]), &c));
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, c, ctx_finalize, NULL, &ext));
+  return ext;
+}
+
+/* ---- geometry ---------------------------------------------------------------- */
+static napi_value js_naca4(napi_env env, napi_callback_info info) {
+  napi_value a[8];
+  if (!get_args(env, info, 8, a)) return NULL;
+  const int M = (int)num(env, a[7]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_naca4(num(env, a[0]), num(env, a[1]), num(env, a[2]), num(env, a[3]),
+                                     num(env, a[4]), num(env, a[5]), (int)num(env, a[6]), M, (double*)d));
+  return out;
+}
+
+static napi_value js_circle(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  const int M = (int)num(env, a[1]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_circle(num(env, a[0]), M, (double*)d));
+  return out;
+}
+
+static napi_value js_rect(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  const int M = (int)num(env, a[1]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_rect(num(env, a[0]), M, (double*)d));
+  return out;
+}
+
+/* ---- grid -------------------------------------------------------------------- */
+static napi_value js_grid_from_boundary(napi_env env, napi_callback_info info) {
+  napi_value a[6];
+  if (!get_args(env, info, 6, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  size_t nb, no;
+  double* body = (double*)typed(env, a[1], napi_float64_array, &nb);
+  double* outer = (double*)typed(env, a[2], napi_float64_array, &no);
+  if (!c || !body || !outer) return NULL;
+  AUSM_CALL(env, ausm_grid_from_boundary(c, body, outer, (int)num(env, a[3]), (int)num(env, a[4]),
+                                         num(env, a[5])));
+  return NULL;
+}
+
+static napi_value js_set_vertices(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  size_t n;
+  float* v = (float*)typed(env, a[1], napi_float32_array, &n);
+  if (!c || !v) return NULL;
+  AUSM_CALL(env, ausm_grid_set_vertices(c, v));
+  return NULL;
+}
+
+static napi_value js_get_vertices(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  const size_t n = (size_t)num(env, a[1]) * ((size_t)num(env, a[2]) + 1) * 2;
+  void* d;
+  napi_value out = new_typed(env, napi_float32_array, n, &d);
+  AUSM_CALL(env, ausm_grid_get_vertices(c, (float*)d));
+  return out;
+}
+
+/* ---- parameters ------------------------------------------------------------------ */
+static napi_value js_set_freestream(napi_env env, napi_callback_info info) {
+  napi_value a[6];
+  if (!get_args(env, info, 6, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  AUSM_CALL(env, ausm_set_freestream(c, num(env, a[1]), num(env, a[2]), num(env, a[3]), num(env, a[4]),
+                                     num(env, a[5])));
+  return NULL;
+}
+
+static napi_value js_set_scheme(napi_env env, napi_callback_info info) {
+  napi_value a[5];
+  if (!get_args(env, info, 5, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  AUSM_CALL(env, ausm_set_scheme(c, (int)num(env, a[1]), (int)num(env, a[2]), (float)num(env, a[3]),
+                                 (float)num(env, a[4])));
+  return NULL;
+}
+
+static napi_value js_set_cfl(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  AUSM_CALL(env, ausm_set_cfl(c, (float)num(env, a[1]), (float)num(env, a[2]), (int)num(env, a[3])));
+  return NULL;
+}
+
+static napi_value js_set_math(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  AUSM_CALL(env, ausm_set_math(c, (int)num(env, a[1])));
+  return NULL;
+}
+
+/* ---- state --------------------------------------------------------------------- */
+static napi_value js_init_state(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  AUSM_CALL(env, ausm_init_state(c, (int)num(env, a[1])));
+  return NULL;
+}
+
+static napi_value js_set_state(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  size_t n;
+  float* q = (float*)typed(env, a[1], napi_float32_array, &n);
+  if (!c || !q) return NULL;
+  AUSM_CALL(env, ausm_set_state(c, q));
+  return NULL;
+}
+
+static napi_value js_get_state(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  const size_t n = (size_t)num(env, a[1]) * ((size_t)num(env, a[2]) + 3) * 4;
+  void* d;
+  napi_value out = new_typed(env, napi_float32_array, n, &d);
+  AUSM_CALL(env, ausm_get_state(c, (float*)d));
+  return out;
+}
+
+static napi_value info_object(napi_env env, const ausm_step_info* si) {
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "dt", si->dt);
+  set_num(env, o, "maxWaveSpeed", si->max_wave_speed);
+  set_num(env, o, "resL2", si->res_l2);
+  set_num(env, o, "nonfinite", si->nonfinite);
+  set_num(env, o, "stepsDone", (double)si->steps_done);
+  return o;
+}
+
+static napi_value js_step(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  ausm_step_info si;
+  memset(&si, 0, sizeof si);
+  AUSM_CALL(env, ausm_step(c, (int)num(env, a[1]), &si));
+  return info_object(env, &si);
+}
+
+/* asynchronous steps on the libuv pool */
+typedef struct {
+  ausm_ctx* ctx;
+  int n;
+  int rc;
+  char err[512];
+  ausm_step_info si;
+  napi_deferred deferred;
+  napi_async_work work;
+} step_job;
+
+static void step_execute(napi_env env, void* data) {
+  (void)env;
+  step_job* j = (step_job*)data;
+  memset(&j->si, 0, sizeof j->si);
+  j->rc = ausm_step(j->ctx, j->n, &j->si);
+  if (j->rc != AUSM_OK) snprintf(j->err, sizeof j->err, "ausm error %d: %s", j->rc, ausm_last_error());
+}
+
+static void step_complete(napi_env env, napi_status status, void* data) {
+  step_job* j = (step_job*)data;
+  if (status == napi_ok && j->rc == AUSM_OK) {
+    napi_resolve_deferred(env, j->deferred, info_object(env, &j->si));
+  } else {
+    napi_value msg, err;
+    napi_create_string_utf8(env, j->rc != AUSM_OK ? j->err : "step cancelled", NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, NULL, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  napi_delete_async_work(env, j->work);
+  free(j);
+}
+
+static napi_value js_step_async(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  step_job* j = (step_job*)calloc(1, sizeof(step_job));
+  j->ctx = c;
+  j->n = (int)num(env, a[1]);
+  napi_value promise, name;
+  NAPI_CALL(env, napi_create_promise(env, &j->deferred, &promise));
+  NAPI_CALL(env, napi_create_string_utf8(env, "ausm_step", NAPI_AUTO_LENGTH, &name));
+  NAPI_CALL(env, napi_create_async_work(env, NULL, name, step_execute, step_complete, j, &j->work));
+  NAPI_CALL(env, napi_queue_async_work(env, j->work));
+  return promise;
+}
+
+static napi_value js_forces(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  const size_t M = (size_t)num(env, a[2]);
+  double out[4];
+  void* cp;
+  napi_value cpa = new_typed(env, napi_float64_array, M, &cp);
+  AUSM_CALL(env, ausm_forces(c, num(env, a[1]), out, (double*)cp));
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "fx", out[0]);
+  set_num(env, o, "fy", out[1]);
+  set_num(env, o, "cl", out[2]);
+  set_num(env, o, "cd", out[3]);
+  napi_set_named_property(env, o, "cp", cpa);
+  return o;
+}
+
+static napi_value js_residual(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  const size_t n = (size_t)num(env, a[1]) * (size_t)num(env, a[2]) * 4;
+  void* d;
+  napi_value out = new_typed(env, napi_float32_array, n, &d);
+  AUSM_CALL(env, ausm_eval_residual(c, (float*)d));
+  return out;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  (void)env;
+  (void)info;
+  /* contexts are released by the external's finalizer */
+  return NULL;
+}
+
+static napi_value js_version(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value v;
+  napi_create_int32(env, ausm_version(), &v);
+  return v;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  const napi_property_descriptor props[] = {
+      {"version", NULL, js_version, NULL, NULL, NULL, napi_default, NULL},
+      {"createContext", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
+      {"destroyContext", NULL, js_destroy, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryNaca4", NULL, js_naca4, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryCircle", NULL, js_circle, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryRect", NULL, js_rect, NULL, NULL, NULL, napi_default, NULL},
+      {"gridFromBoundary", NULL, js_grid_from_boundary, NULL, NULL, NULL, napi_default, NULL},
+      {"setVertices", NULL, js_set_vertices, NULL, NULL, NULL, napi_default, NULL},
+      {"getVertices", NULL, js_get_vertices, NULL, NULL, NULL, napi_default, NULL},
+      {"setFreestream", NULL, js_set_freestream, NULL, NULL, NULL, napi_default, NULL},
+      {"setScheme", NULL, js_set_scheme, NULL, NULL, NULL, napi_default, NULL},
+      {"setCfl", NULL, js_set_cfl, NULL, NULL, NULL, napi_default, NULL},
+      {"setMath", NULL, js_set_math, NULL, NULL, NULL, napi_default, NULL},
+      {"initState", NULL, js_init_state, NULL, NULL, NULL, napi_default, NULL},
+      {"setState", NULL, js_set_state, NULL, NULL, NULL, napi_default, NULL},
+      {"getState", NULL, js_get_state, NULL, NULL, NULL, napi_default, NULL},
+      {"step", NULL, js_step, NULL, NULL, NULL, napi_default, NULL},
+      {"stepAsync", NULL, js_step_async, NULL, NULL, NULL, napi_default, NULL},
+      {"forces", NULL, js_forces, NULL, NULL, NULL, napi_default, NULL},
+      {"residual", NULL, js_residual, NULL, NULL, NULL, napi_default, NULL},
+  };
+  napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

@@ -13,7 +13,15 @@ Prints one JSON line: cell-steps/s over K timed steps (inputs resident in
 HBM), the stage kernel's roofline position from HIP events around
 individual launches, and the CPU baseline (the FP32 oracle in the
 reference's unfused 15-pass structure, OpenMP over rows) on a bounded
-sample of the same workload.
+sample of the same workload.  Two more objects ride on the same line:
+  batched  64 solves with sizes drawn from {256x128, 384x192, 512x384,
+           768x512}, random NACA 4-digit sections, Mach U[0.3,3], AoA
+           U[-4,8] deg, scheme uniform over SLAU2/SLAU/AUSM+-up, 5000 steps
+           each: persistent batched kernel vs the same solves one by one;
+  serving  Poisson arrivals of requests with sizes 128x64 ... 1024x768, Mach
+           log-U[0.1,10], AoA U[-5,10] deg, each run to a residual drop or a
+           step cap, served with continuous batching; converged solves/s and
+           p50/p99 time-to-converged against the one-by-one baseline.
 """
 import argparse
 import ctypes
@@ -150,6 +158,85 @@ def cpu_baseline(vtx, M, N, target_s=15.0):
             "steps_per_s": n / dt}
 
 
+def probe(ausm, ev, M, N, args):
+    H = ev.L
+    H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    H.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    H.hipFree.argtypes = [ctypes.c_void_p]
+    n = 512 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert H.hipMalloc(ctypes.byref(a), n) == 0 and H.hipMalloc(ctypes.byref(b), n) == 0
+    assert H.hipMemset(a, 1, n) == 0
+    for _ in range(3):
+        assert H.hipMemcpy(b, a, n, 3) == 0  # device to device
+    ev.device_sync()
+    H.hipFree(a)
+    H.hipFree(b)
+    s, _ = build_workload(ausm, M, N, args.grid_iters)
+    s.set_math(args.math)
+    s.prepare_state(freestream=True)
+    s.step(args.steps, info=False)
+    s.sync()
+    log(f"[bench] probe: 3 x {n} B copies, {args.steps} steps")
+    s.close()
+
+
+def batched_leg(serve, n, steps, grid_iters):
+    sizes = [(256, 128), (384, 192), (512, 384), (768, 512)]
+    reqs = serve.make_requests(n, seed=2026, sizes=sizes, max_steps=steps, res_drop=0.0)
+    serve.make_grids(reqs, iters=grid_iters)
+    log(f"[bench] batched: {n} solves x {steps} steps")
+    bat = serve.run_batched(reqs, None, check_every=250, max_solves=n)
+    log(f"[bench] batched {bat['solves_per_s']:.2f} solves/s, {bat['cell_steps_per_s']:.3e} cell-steps/s")
+    seq = serve.run_sequential(reqs, None, check_every=250)
+    log(f"[bench] one by one {seq['solves_per_s']:.2f} solves/s, {seq['cell_steps_per_s']:.3e} cell-steps/s")
+    same = all(bat["results"][r["id"]]["state"] == seq["results"][r["id"]]["state"] and
+               (bat["results"][r["id"]]["state"] == "failed" or
+                bat["results"][r["id"]]["res_l2"] == seq["results"][r["id"]]["res_l2"])
+               for r in reqs)
+    strip = lambda d: {k: v for k, v in d.items() if k != "results"}
+    return {"workload": f"{n} solves, sizes from {sizes}, NACA m~U[0,.06] p~U[.2,.6] "
+                        f"t~U[.08,.18], Mach~U[0.3,3], AoA~U[-4,8], scheme~U{{SLAU2,SLAU,AUSM+-up}}, "
+                        f"{steps} steps each",
+            "batched": strip(bat), "one_by_one": strip(seq),
+            "speedup_solves_per_s": bat["solves_per_s"] / seq["solves_per_s"],
+            "results_identical": same}
+
+
+def serving_leg(serve, n, cap, drop, rates, grid_iters):
+    sizes = [(128, 64), (256, 128), (384, 192), (512, 384), (768, 512), (1024, 768)]
+    reqs = serve.make_requests(n, seed=5, sizes=sizes, mach_range=(0.1, 10.0), log_mach=True,
+                               aoa_range=(-5.0, 10.0), max_steps=cap, res_drop=drop)
+    serve.make_grids(reqs, iters=grid_iters)
+    strip = lambda d: {k: v for k, v in d.items() if k != "results"}
+    out = {"workload": f"{n} requests, sizes from {sizes}, Mach~logU[0.1,10], AoA~U[-5,10], "
+                       f"residual drop {drop:g} or {cap}-step cap, checks every 100 steps",
+           "rates": []}
+    seq = serve.run_sequential(reqs, None, check_every=100)
+    out["one_by_one_closed_loop"] = strip(seq)
+    log(f"[bench] serving one by one: {seq['solves_per_s']:.2f} solves/s "
+        f"({seq['converged']} converged)")
+    for rate in rates:
+        arr = None if rate == 0 else serve.poisson_arrivals(n, rate, seed=11)
+        r = serve.run_batched(reqs, arr, check_every=100, max_solves=64)
+        rec = dict(strip(r), arrival_rate_per_s=("all at t=0" if rate == 0 else rate))
+        if rate:
+            sq = serve.run_sequential(reqs, arr, check_every=100)
+            rec["one_by_one_p50_s"] = sq["p50_latency_s"]
+            rec["one_by_one_p99_s"] = sq["p99_latency_s"]
+        out["rates"].append(rec)
+        log(f"[bench] serving rate {rate}: {r['solves_per_s']:.2f} solves/s, "
+            f"{r['converged_solves_per_s']:.2f} converged/s, p50 {r['p50_latency_s']:.3f} s, "
+            f"p99 {r['p99_latency_s']:.3f} s")
+    sat = out["rates"][0]
+    out["speedup_converged_solves_per_s"] = (
+        sat["converged_solves_per_s"] / seq["converged_solves_per_s"]
+        if seq["converged_solves_per_s"] > 0 else None)
+    out["speedup_solves_per_s"] = sat["solves_per_s"] / seq["solves_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +245,18 @@ def main():
     ap.add_argument("--grid-iters", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--math", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-solves", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-cap", type=int, default=20000)
+    ap.add_argument("--serve-drop", type=float, default=1e-5)
+    ap.add_argument("--serve-rates", default="0,4")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter-collection run: three 512 MiB device copies (the calibration "
+                         "kernel for FETCH_SIZE / WRITE_SIZE) then --steps steps of the workload, "
+                         "no timing and no JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -170,6 +269,9 @@ def main():
 
     M, N = 512, 384
     ev = HipEvents()
+    if args.probe:
+        probe(ausm, ev, M, N, args)
+        return
     log(f"[bench] building {M}x{N} NACA 2412 grid ({args.grid_iters} FP64 Jacobi sweeps)")
     s, t_grid = build_workload(ausm, M, N, args.grid_iters)
     log(f"[bench] grid in {t_grid:.2f} s")
@@ -203,6 +305,17 @@ def main():
         cpu = cpu_baseline(s.vertices(), M, N)
         log(f"[bench] CPU: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
 
+    from ausm import serve
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(serve, args.batch_solves, args.batch_steps, 2000)
+    serving = None
+    if not args.no_serving:
+        rates = [float(x) for x in args.serve_rates.split(",") if x != ""]
+        serving = serving_leg(serve, args.serve_requests, args.serve_cap, args.serve_drop, rates,
+                              2000)
+
     whole_gbps = ALG_BYTES_PER_CELL_STEP * rate / 1e9
     out = {
         "metric": METRIC,
@@ -230,6 +343,8 @@ def main():
                      "kernel": "ausm::stage_kernel (fused RK stage)",
                      "stage_us": [x * 1e3 for x in stage_ms]},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     s.close()

@@ -119,8 +119,10 @@ int ausm_set_freestream(ausm_ctx* ctx, double rho, double p, double vx, double v
                         double gamma);
 int ausm_set_scheme(ausm_ctx* ctx, int scheme, int limiter, float K_p, float K_u);
 /* dt = min(maxdt, cfl / max wave speed); the wave speed is refreshed from
- * the solution after every `steps_per_dt_refresh` steps (the reference
- * refreshes once per frame, main.js:789-791). */
+ * the solution after every `steps_per_dt_refresh` steps counted from the
+ * last refresh (the reference refreshes once per frame of stepsPerFrame
+ * steps, main.js:789-791; a host that varies stepsPerFrame sets it before
+ * each frame). */
 int ausm_set_cfl(ausm_ctx* ctx, float cfl, float maxdt, int steps_per_dt_refresh);
 int ausm_set_math(ausm_ctx* ctx, int math);
 /* rows per wavefront tile of the stage kernel (0 = autotuned) */
