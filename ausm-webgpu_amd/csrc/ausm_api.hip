@@ -143,10 +143,12 @@ StageArgs stage_args(ausm_ctx* c, int ty, int stage) {
 }
 
 // Enqueue one TVD-RK3 step: three fused stage launches (the reference's 15
-// passes, main.js:744-777).
-hipError_t enqueue_step(ausm_ctx* c, int ty, bool fresh, bool refresh) {
+// passes, main.js:744-777).  `norm`: stage 1 also writes the partial sums of
+// the density-residual norm (only the last step of a call needs them).
+hipError_t enqueue_step(ausm_ctx* c, int ty, bool fresh, bool refresh, bool norm) {
   StageArgs a = stage_args(c, ty, 1);
   a.fresh = fresh;
+  if (!norm) a.res_out = nullptr;
   a.qin = c->q[0];
   a.qout = c->q[1];
   hipError_t e = launch_stage(a, c->scheme, c->math, ty, c->stream);
@@ -620,7 +622,7 @@ int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
         for (int s = 0; s < G && e == hipSuccess; ++s) {
           const bool fresh = (s % K) == 0;
           const bool refresh = ((s + 1) % K) == 0;
-          e = enqueue_step(c, ty, fresh, refresh);
+          e = enqueue_step(c, ty, fresh, refresh, s == G - 1);
           if (refresh) c->n_ws_valid = stage_blocks(c->M, c->N, ty);
         }
         hipGraph_t graph;
@@ -641,7 +643,7 @@ int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
     } else {
       const bool fresh = c->ws_fresh;
       const bool refresh = c->since_refresh + 1 >= K;
-      CHECK_HIP(enqueue_step(c, ty, fresh, refresh), "stage launch");
+      CHECK_HIP(enqueue_step(c, ty, fresh, refresh, left == 1), "stage launch");
       c->ws_fresh = refresh;
       c->since_refresh = refresh ? 0 : c->since_refresh + 1;
       if (refresh) c->n_ws_valid = stage_blocks(c->M, c->N, ty);
@@ -751,7 +753,7 @@ int ausm_profile_stages(ausm_ctx* c, int reps, float* ms3) {
     a.qout = st == 2 ? c->q[2] : c->q[1];  // scratch outputs only
     a.fresh = 0;
     a.refresh = st == 3;
-    a.ws_out = c->ws_part + 0;  // refresh writes partials; restored below
+    a.res_out = nullptr;  // the residual norm is formed on check steps only
     double total = 0.0;
     for (int k = 0; k < reps; ++k) {
       CHECK_HIP(hipEventRecord(e0, c->stream), "event");

@@ -352,7 +352,7 @@ static int admit(ausm_batch* b) {
 
 // time one tuning stage launch over the current items
 static float time_schedule(ausm_batch* b, int ty, int bpc) {
-  BatchArgs a{b->d_solves, b->d_items, b->d_counts, 4, 0};
+  BatchArgs a{b->d_solves, b->d_items, b->d_counts, 4, 0, 0};
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
@@ -425,7 +425,8 @@ static hipError_t enqueue_steps(ausm_batch* b, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
     hipError_t e = launch_batch_dt(b->d_solves, b->d_counts + 1, (int)b->table.size(), b->stream);
     for (int st = 1; st <= 3 && e == hipSuccess; ++st) {
-      BatchArgs a{b->d_solves, b->d_items, b->d_counts, st, st == 3};
+      // the convergence check after the chunk reads the last step's norm
+      BatchArgs a{b->d_solves, b->d_items, b->d_counts, st, st == 3, st == 1 && s == nsteps - 1};
       e = launch_batch_stage(a, b->math, b->ty, blocks, b->stream);
     }
     if (e != hipSuccess) return e;

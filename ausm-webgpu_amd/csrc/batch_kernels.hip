@@ -20,14 +20,15 @@
 
 namespace ausm {
 
-__device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, int refresh) {
+__device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, int refresh,
+                                                int norm) {
   StageArgs a{};
   a.vx = S.vx;
   a.vy = S.vy;
   a.par = nullptr;
   a.ws_in = S.ws_part;
   a.ws_out = S.ws_part;
-  a.res_out = S.res_part;
+  a.res_out = norm ? S.res_part : nullptr;
   a.maxws_cur = S.maxws_cur;
   a.n_ws_in = S.n_ws;
   a.M = S.M;
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
   for (int it = lo; it < hi; ++it) {
     const BatchItem item = b.items[it];
     const SolveDev* S = b.solves + item.solve;
-    const StageArgs a = batch_args(*S, b.stage, b.refresh);
+    const StageArgs a = batch_args(*S, b.stage, b.refresh, b.norm);
     const DevParams P = S->par;
     const int tx = item.tile % S->tiles_x;
     const int tyb = item.tile / S->tiles_x;

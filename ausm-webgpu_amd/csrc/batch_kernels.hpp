@@ -34,6 +34,7 @@ struct BatchArgs {
   const int* n_items;  // device-resident so that captured graphs serve any mix
   int stage;           // 1, 2, 3; 4 = schedule tuning (no state change)
   int refresh;
+  int norm;            // stage 1: write the residual-norm partials
 };
 
 hipError_t launch_batch_stage(const BatchArgs& b, int fast, int ty, int blocks, hipStream_t s);

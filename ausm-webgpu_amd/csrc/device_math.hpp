@@ -10,8 +10,11 @@
 //    right, normalize(v) = v/length(v), mix(a,b,t) = a*(1-t)+b*t,
 //    reflect(e,n) = e-(2*dot(n,e))*n, step(e,x) = x>=e, sign(0) = 0.
 //  * Fast — the same scheme rearranged for CDNA4: shared reciprocals
-//    (v_rcp_f32), hardware sqrt, one reciprocal per limiter ratio pair and
-//    FMA contraction.  It differs from Exact by a few ulp per operation.
+//    (v_rcp_f32), hardware sqrt, one reciprocal per limiter ratio and
+//    explicit fused multiply-adds.  It differs from Exact by a few ulp per
+//    operation.  Fusion is written out (fmaf), never left to the compiler's
+//    contraction, so every kernel that inlines these functions (any tile
+//    shape, single grid or batch) produces the same bits.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -62,13 +65,13 @@ __device__ __forceinline__ Prim to_prim_exact(Cons q, float gamma) {
 }
 
 __device__ __forceinline__ Prim to_prim_fast(Cons q, float gm1) {
-#pragma clang fp contract(fast)
   Prim w;
   const float inv = __builtin_amdgcn_rcpf(q.r);
   w.r = q.r;
   w.u = q.mu * inv;
   w.v = q.mv * inv;
-  const float p = (q.e - 0.5f * (q.mu * w.u + q.mv * w.v)) * gm1;
+  const float ke = fmaf(q.mu, w.u, q.mv * w.v);
+  const float p = fmaf(-0.5f, ke, q.e) * gm1;
   w.p = fmaxf(1e-7f, p);
   return w;
 }
@@ -82,8 +85,7 @@ __device__ __forceinline__ void normal_exact(float dx, float dy, float& nx, floa
   ny = tx;
 }
 __device__ __forceinline__ void normal_fast(float dx, float dy, float& nx, float& ny, float& len) {
-#pragma clang fp contract(fast)
-  const float dd = dx * dx + dy * dy;
+  const float dd = fmaf(dx, dx, dy * dy);
   const float rs = __builtin_amdgcn_rsqf(dd);
   len = dd * rs;
   nx = -dy * rs;
@@ -234,40 +236,39 @@ __device__ __forceinline__ Flux face_flux_exact(const DevParams& P, int limiter,
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void muscl_fast(int limiter, float wl2, float wl, float wr, float wr2,
                                            float& L, float& Rr) {
-#pragma clang fp contract(fast)
   const float eps = 1e-6f;
   const float dL = wl - wl2;
   const float dF = wr - wl;
   const float dR = wr2 - wr;
+  const float hF = 0.5f * dF;
   if (limiter == LIM_MINMOD) {
     const float mL = (dL * dF > 0.0f) ? copysignf(fminf(fabsf(dL), fabsf(dF)), dF) : 0.0f;
     const float mR = (dF * dR > 0.0f) ? copysignf(fminf(fabsf(dF), fabsf(dR)), dF) : 0.0f;
-    L = wl + 0.5f * mL;
-    Rr = wr - 0.5f * mR;
+    L = fmaf(0.5f, mL, wl);
+    Rr = fmaf(-0.5f, mR, wr);
   } else if (limiter == LIM_VANLEER) {
     // phi(r) = (r+|r|)/(1+|r|) with r = n/d, d > 0:  (n+|n|)/(d+|n|)
-    const float d = dF * dF + eps;
-    const float nL = dL * dF + eps, nR = dF * dR + eps;
+    const float d = fmaf(dF, dF, eps);
+    const float nL = fmaf(dL, dF, eps), nR = fmaf(dF, dR, eps);
     const float phiL = (nL + fabsf(nL)) * __builtin_amdgcn_rcpf(d + fabsf(nL));
     const float phiR = (nR + fabsf(nR)) * __builtin_amdgcn_rcpf(d + fabsf(nR));
-    L = wl + 0.5f * phiL * dF;
-    Rr = wr - 0.5f * phiR * dF;
+    L = fmaf(phiL, hF, wl);
+    Rr = fmaf(-phiR, hF, wr);
   } else {
     // van Albada phi(r) = (r^2+r)/(r^2+1) with r = n/d:  n(n+d)/(n^2+d^2)
-    const float d = dF * dF + eps;
+    const float d = fmaf(dF, dF, eps);
     const float d2 = d * d;
-    const float nL = dL * dF + eps, nR = dF * dR + eps;
-    const float phiL = nL * (nL + d) * __builtin_amdgcn_rcpf(nL * nL + d2);
-    const float phiR = nR * (nR + d) * __builtin_amdgcn_rcpf(nR * nR + d2);
-    L = wl + 0.5f * phiL * dF;
-    Rr = wr - 0.5f * phiR * dF;
+    const float nL = fmaf(dL, dF, eps), nR = fmaf(dF, dR, eps);
+    const float phiL = nL * (nL + d) * __builtin_amdgcn_rcpf(fmaf(nL, nL, d2));
+    const float phiR = nR * (nR + d) * __builtin_amdgcn_rcpf(fmaf(nR, nR, d2));
+    L = fmaf(phiL, hF, wl);
+    Rr = fmaf(-phiR, hF, wr);
   }
 }
 
 template <int SCHEME>
 __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, Prim a, Prim b,
                                                Prim c, Prim d, float nx, float ny) {
-#pragma clang fp contract(fast)
   const float gamma = P.gamma;
   Prim L = b, R = c;
   if (limiter != LIM_NONE) {
@@ -276,16 +277,17 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     muscl_fast(limiter, a.v, b.v, c.v, d.v, L.v, R.v);
     muscl_fast(limiter, a.p, b.p, c.p, d.p, L.p, R.p);
   }
-  const float u_L = L.u * nx + L.v * ny;
-  const float u_R = R.u * nx + R.v * ny;
-  const float vvL = L.u * L.u + L.v * L.v;
-  const float vvR = R.u * R.u + R.v * R.v;
+  const float u_L = fmaf(L.u, nx, L.v * ny);
+  const float u_R = fmaf(R.u, nx, R.v * ny);
+  const float vvL = fmaf(L.u, L.u, L.v * L.v);
+  const float vvR = fmaf(R.u, R.u, R.v * R.v);
   const float rcL = fmaxf(L.r, 1e-10f), rcR = fmaxf(R.r, 1e-10f);
   const float iclL = __builtin_amdgcn_rcpf(rcL), iclR = __builtin_amdgcn_rcpf(rcR);
   const float irL = L.r > 1e-10f ? iclL : __builtin_amdgcn_rcpf(L.r);
   const float irR = R.r > 1e-10f ? iclR : __builtin_amdgcn_rcpf(R.r);
-  const float h_L = P.ggm1 * (L.p * irL) + 0.5f * vvL;
-  const float h_R = P.ggm1 * (R.p * irR) + 0.5f * vvR;
+  // h = p / ((gamma-1) rho) + |v|^2 / 2 + p / rho = gamma/(gamma-1) p/rho + |v|^2 / 2
+  const float h_L = fmaf(P.ggm1, L.p * irL, 0.5f * vvL);
+  const float h_R = fmaf(P.ggm1, R.p * irR, 0.5f * vvR);
   float mdot, P_i;
   if constexpr (SCHEME == SCHEME_AUSMUP) {
     const float M_inf2 = P.minf_coef * L.r;
@@ -299,7 +301,7 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float M_L = u_L * ia;
     const float M_R = u_R * ia;
     const float a_i2 = a_i * a_i;
-    const float Mbar2 = (u_L * u_L + u_R * u_R) * (0.5f * ia * ia);
+    const float Mbar2 = fmaf(u_L, u_L, u_R * u_R) * (0.5f * ia * ia);
     const float M_o2 = fminf(1.0f, fmaxf(Mbar2, M_inf2));
     const float f_a = M_o2 * (2.0f - M_o2);
     const float rho_i = 0.5f * (L.r + R.r);
@@ -308,21 +310,21 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float M2pL = 0.25f * Mp1L * Mp1L, M2pR = 0.25f * Mp1R * Mp1R;
     const float M2mL = -0.25f * Mm1L * Mm1L, M2mR = -0.25f * Mm1R * Mm1R;
     const bool supL = fabsf(M_L) >= 1.0f, supR = fabsf(M_R) >= 1.0f;
-    const float M4L = supL ? fmaxf(0.0f, M_L) : M2pL * (1.0f + 2.0f * M2mL);
-    const float M4R = supR ? fminf(0.0f, M_R) : M2mR * (1.0f - 2.0f * M2pR);
+    const float M4L = supL ? fmaxf(0.0f, M_L) : M2pL * fmaf(2.0f, M2mL, 1.0f);
+    const float M4R = supR ? fminf(0.0f, M_R) : M2mR * fmaf(-2.0f, M2pR, 1.0f);
     float M_i = M4L + M4R;
     if (P.Kp != 0.0f) {
       const float kp = P.Kp * fmaxf(1.0f - Mbar2, 0.0f) * (R.p - L.p) *
                        __builtin_amdgcn_rcpf(f_a * rho_i * a_i2);
       M_i -= kp;
     }
-    const float alpha4 = 0.75f * (-4.0f + 5.0f * f_a * f_a);
+    const float alpha4 = 0.75f * fmaf(5.0f * f_a, f_a, -4.0f);
     const float P5L = supL ? (M_L >= 0.0f ? 1.0f : 0.0f)
-                           : M2pL * ((2.0f - M_L) + alpha4 * M_L * M2mL);
+                           : M2pL * fmaf(alpha4 * M_L, M2mL, 2.0f - M_L);
     const float P5R = supR ? (-M_R >= 0.0f ? 1.0f : 0.0f)
-                           : M2mR * ((-2.0f - M_R) - alpha4 * M_R * M2pR);
+                           : M2mR * fmaf(-alpha4 * M_R, M2pR, -2.0f - M_R);
     const float ku = P.Ku * P5L * P5R * (L.r + R.r) * f_a * a_i * (u_R - u_L);
-    P_i = (P5L * L.p + P5R * R.p) - ku;
+    P_i = fmaf(P5L, L.p, P5R * R.p) - ku;
     mdot = a_i * M_i * (M_i > 0.0f ? L.r : R.r);
   } else {
     const float a_L = __builtin_amdgcn_sqrtf(gamma * fmaxf(L.p, 1e-10f) * iclL);
@@ -338,26 +340,26 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float qL = M_L + 1.0f, qR = M_R - 1.0f;
     const float fpp = wsat(supL ? 0.5f * (1.0f + wsign(M_L)) : 0.25f * qL * qL * (2.0f - M_L));
     const float fpm = wsat(supR ? 0.5f * (1.0f - wsign(M_R)) : 0.25f * qR * qR * (2.0f + M_R));
+    const float fsum = (fpp + fpm) - 1.0f;
     if constexpr (SCHEME == SCHEME_SLAU2) {
-      P_i = 0.5f * ((fpp - fpm) * (L.p - R.p) + (L.p + R.p) +
-                    Uarc * ((fpp + fpm) - 1.0f) * (L.r + R.r) * a_i);
+      P_i = 0.5f * fmaf(fpp - fpm, L.p - R.p, fmaf(Uarc * fsum, (L.r + R.r) * a_i, L.p + R.p));
     } else {
-      P_i = 0.5f * ((fpp - fpm) * (L.p - R.p) +
-                    (1.0f + (1.0f - X) * ((fpp + fpm) - 1.0f)) * (L.p + R.p));
+      P_i = 0.5f * fmaf(fpp - fpm, L.p - R.p, fmaf(1.0f - X, fsum, 1.0f) * (L.p + R.p));
     }
     const float delta_p = R.p - L.p;
     const float ubar_n =
-        (L.r * fabsf(u_L) + R.r * fabsf(u_R)) * __builtin_amdgcn_rcpf(L.r + R.r);
+        fmaf(L.r, fabsf(u_L), R.r * fabsf(u_R)) * __builtin_amdgcn_rcpf(L.r + R.r);
     const float g = wsat(-M_L) * wsat(M_R);
-    const float ubp = ubar_n + g * (fabsf(u_L) - ubar_n);
-    const float ubm = ubar_n + g * (fabsf(u_R) - ubar_n);
-    mdot = 0.5f * ((L.r * (u_L + ubp) + R.r * (u_R - ubm)) * (1.0f - g) - X * ia * delta_p);
+    const float ubp = fmaf(g, fabsf(u_L) - ubar_n, ubar_n);
+    const float ubm = fmaf(g, fabsf(u_R) - ubar_n, ubar_n);
+    const float conv = fmaf(L.r, u_L + ubp, R.r * (u_R - ubm));
+    mdot = 0.5f * fmaf(conv, 1.0f - g, -(X * ia) * delta_p);
   }
   const bool left = mdot > 0.0f;
   Flux F;
   F.f0 = mdot;
-  F.f1 = (left ? L.u : R.u) * mdot + nx * P_i;
-  F.f2 = (left ? L.v : R.v) * mdot + ny * P_i;
+  F.f1 = fmaf(left ? L.u : R.u, mdot, nx * P_i);
+  F.f2 = fmaf(left ? L.v : R.v, mdot, ny * P_i);
   F.f3 = (left ? h_L : h_R) * mdot;
   return F;
 }

@@ -106,26 +106,38 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
         qnr[k] = Cons{a.qn[o], a.qn[PS + o], a.qn[2 * PS + o], a.qn[3 * PS + o]};
       }
     }
-    // body / far-field segment normals for tiles whose window reaches a ghost
-    float bnx = 0.0f, bny = 0.0f, fnx = 0.0f, fny = 0.0f;
+    // ghost rows, formed once per wavefront when its window reaches them:
+    // the body ghost (rows -1 and -2 of the window) from interior row 1, the
+    // two far-field ghosts from interior row N (ausm.js:57-117)
+    auto to_prim = [&](const Cons& q) -> Prim {
+      if constexpr (FAST) return to_prim_fast(q, P.gm1);
+      else return to_prim_exact(q, P.gamma);
+    };
+    auto row_of = [&](int j) -> Cons {
+      const int o = j * Pp + col;
+      return Cons{qin[o], qin[PS + o], qin[2 * PS + o], qin[3 * PS + o]};
+    };
+    Prim gW{}, gF1{}, gF2{};
     if (j0 < 2) {
-      float len;
-      normal_exact(a.vx[col] - a.vx[colr], a.vy[col] - a.vy[colr], bnx, bny, len);
+      float nx, ny, len;
+      normal_exact(a.vx[col] - a.vx[colr], a.vy[col] - a.vy[colr], nx, ny, len);
+      gW = to_prim(wall_ghost(row_of(0), nx, ny));
     }
     if (j0 + TY + 2 > N) {
       const int o = N * Pp;
-      float len;
-      normal_exact(a.vx[o + col] - a.vx[o + colr], a.vy[o + col] - a.vy[o + colr], fnx, fny, len);
+      float nx, ny, len;
+      normal_exact(a.vx[o + col] - a.vx[o + colr], a.vy[o + col] - a.vy[o + colr], nx, ny, len);
+      const Cons I = row_of(N - 1);
+      gF1 = to_prim(far_ghost(P, I, nx, ny, false));
+      gF2 = to_prim(far_ghost(P, I, nx, ny, true));
     }
-    // primitive state of window slot k (row j0-2+k): interior rows convert
-    // the prefetched row, ghost rows are formed from rows 0 / N-1 first
+    // primitive state of window slot k (row j0-2+k)
     auto prim_of = [&](int k) -> Prim {
       const int j = j0 - 2 + k;
-      Cons q = raw[k];
-      if (j < 0) q = wall_ghost(q, bnx, bny);                      // ausm.js:57-64
-      else if (j >= N) q = far_ghost(P, q, fnx, fny, j == N + 1);  // ausm.js:65-117
-      if constexpr (FAST) return to_prim_fast(q, P.gm1);
-      else return to_prim_exact(q, P.gamma);
+      if (j < 0) return gW;
+      if (j == N) return gF1;
+      if (j > N) return gF2;
+      return to_prim(raw[k]);
     };
 
     const Prim w_m2 = prim_of(0);
@@ -213,7 +225,12 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
             for (int k = 0; k < 4; ++k) nq[k] = 0.75f * qn[k] + 0.25f * (cq[k] + dt * res[k]);
           } else {
             // Qn+1 = (Qn + 2 (Q2 + dt L(Q2))) / 3  (ausm.js:525)
-            for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + 2.0f * (cq[k] + dt * res[k])) / 3.0f;
+            if constexpr (FAST) {
+              for (int k = 0; k < 4; ++k)
+                nq[k] = fmaf(2.0f, fmaf(dt, res[k], cq[k]), qn[k]) * (1.0f / 3.0f);
+            } else {
+              for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + 2.0f * (cq[k] + dt * res[k])) / 3.0f;
+            }
           }
         }
         a.qout[o] = nq[0];
@@ -222,15 +239,26 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
         a.qout[3 * PS + o] = nq[3];
         if (stage == 3 && a.refresh) {
           // wave speed (|u| + a) * (l_L + l_R + l_U + l_D) / (2 A)  (ausm.js:617-630)
-          const float rho = nq[0];
-          const float vx = nq[1] / rho, vy = nq[2] / rho;
-          const float vv = vx * vx + vy * vy;
-          const float p = (nq[3] - (0.5f * rho) * vv) * (P.gamma - 1.0f);
-          const float T = p / rho;
-          const float c = sqrtf(P.gamma * T);
-          const float speed = sqrtf(vv);
-          const float S = ((lL + lR) + lU) + lD;
-          const float ws = ((speed + c) * S) / (2.0f * area);
+          float ws;
+          if constexpr (FAST) {
+            const float ir = __builtin_amdgcn_rcpf(nq[0]);
+            const float vx = nq[1] * ir, vy = nq[2] * ir;
+            const float vv = fmaf(vx, vx, vy * vy);
+            const float p = fmaf(-0.5f * nq[0], vv, nq[3]) * P.gm1;
+            const float c = __builtin_amdgcn_sqrtf(P.gamma * p * ir);
+            const float speed = __builtin_amdgcn_sqrtf(vv);
+            ws = (speed + c) * (((lL + lR) + lU) + lD) * __builtin_amdgcn_rcpf(2.0f * area);
+          } else {
+            const float rho = nq[0];
+            const float vx = nq[1] / rho, vy = nq[2] / rho;
+            const float vv = vx * vx + vy * vy;
+            const float p = (nq[3] - (0.5f * rho) * vv) * (P.gamma - 1.0f);
+            const float T = p / rho;
+            const float c = sqrtf(P.gamma * T);
+            const float speed = sqrtf(vv);
+            const float S = ((lL + lR) + lU) + lD;
+            ws = ((speed + c) * S) / (2.0f * area);
+          }
           ws_acc = nan_max(ws_acc, ws);
         }
       }

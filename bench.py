@@ -316,6 +316,12 @@ def main():
         serving = serving_leg(serve, args.serve_requests, args.serve_cap, args.serve_drop, rates,
                               2000)
 
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
+
     whole_gbps = ALG_BYTES_PER_CELL_STEP * rate / 1e9
     out = {
         "metric": METRIC,
@@ -339,7 +345,9 @@ def main():
         "steps_per_s": args.steps / (ms / 1e3),
         "exact_math_cell_steps_per_s": exact_rate,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_source": "profiles/stage_kernel_traffic.json (rocprofv3 --pmc "
+                                       "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, bytes per launch)",
                      "kernel": "ausm::stage_kernel (fused RK stage)",
                      "stage_us": [x * 1e3 for x in stage_ms]},
         "cpu_baseline": cpu,

@@ -25,8 +25,9 @@ CASES = [
 ]
 
 
-def single_run(c, vtx, steps, math):
+def single_run(c, vtx, steps, math, tile=0):
     s = ausm.Solver(c["M"], c["N"])
+    s.set_tile(tile)
     s.set_vertices(vtx)
     s.set_flow(c["mach"], c["aoa"])
     s.set_scheme(c["scheme"], ausm.VANALBADA, 0.0, 0.75)
@@ -39,11 +40,14 @@ def single_run(c, vtx, steps, math):
     return out
 
 
-@pytest.mark.parametrize("math", ["exact", "fast"])
-def test_batch_matches_single_grid_bitwise(math):
+@pytest.mark.parametrize("math,sched,tile", [("exact", (0, 0), 0), ("fast", (0, 0), 0),
+                                             ("fast", (4, 2), 1), ("fast", (1, 1), 2)])
+def test_batch_matches_single_grid_bitwise(math, sched, tile):
+    """A batch with any schedule equals single-grid runs with any tile shape."""
     steps = 60
     grids = [grid(c["M"], c["N"], c["code"]) for c in CASES]
     with ausm.Batch(max_solves=8, math=math) as b:
+        b.set_schedule(*sched)
         ids = [b.submit(g, ausm.freestream(c["mach"], c["aoa"]), scheme=c["scheme"],
                         max_steps=steps) for c, g in zip(CASES, grids)]
         b.advance(steps, check_every=20)
@@ -51,7 +55,7 @@ def test_batch_matches_single_grid_bitwise(math):
             st = b.poll(sid)
             assert st["state"] == "capped" and st["steps"] == steps
             got = b.state(sid)
-            ref = single_run(c, g, steps, math)
+            ref = single_run(c, g, steps, math, tile)
             N = c["N"]
             assert np.array_equal(got[1:N + 1], ref[1:N + 1]), c
 

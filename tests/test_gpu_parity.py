@@ -284,18 +284,23 @@ def test_state_roundtrip_and_ghost_rows():
         s.close()
 
 
-def test_tiles_graphs_and_reruns_identical():
+@pytest.mark.parametrize("math_mode", ["exact", "fast"])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_tiles_graphs_and_reruns_identical(math_mode, scheme):
+    """Every tile shape (and graph replay vs direct launches) gives the same
+    bits, in both math modes: results never depend on the autotuner."""
     M, N = 150, 37
     vtx, _, _ = make_grid(M, N, code=4412, iters=100)
     outs = []
     for tile in (1, 2, 4, 8):
-        s, _, par = new_pair(vtx, 0.85, 1.0, SLAU2, VANALBADA, tile=tile)
+        s, _, par = new_pair(vtx, 0.85, 1.0, scheme, VANALBADA, math_mode=math_mode, tile=tile)
         s.set_state(perturbed_state(par, M, N, seed=4))
-        s.step(70)
-        outs.append(s.state())
+        info = s.step(70)
+        outs.append((s.state(), info["res_l2"]))
         s.close()
-    for o in outs[1:]:
-        assert np.array_equal(o, outs[0])
+    for o, r in outs[1:]:
+        assert np.array_equal(o, outs[0][0])
+        assert r == outs[0][1]
 
 
 def test_small_grid_wraps_within_a_wavefront():
