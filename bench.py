@@ -261,7 +261,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the solver step has no multi-GPU path in this "
-              "build (batched solves shard by solve, one process per GPU; see DESIGN.md).",
+              "build; batched solves shard by solve, one process per GPU.",
               file=sys.stderr)
         sys.exit(2)
 
