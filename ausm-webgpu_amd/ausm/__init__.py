@@ -1,7 +1,8 @@
 """ausm: MI355X-native explicit O-grid solver step (SLAU2 / SLAU / AUSM+-up,
 MUSCL, TVD-RK3) behind the C ABI of include/ausm.h."""
 from .lib import (AUSMUP, GRID_JACOBI, GRID_SOR, INIT_FREESTREAM, INIT_REST, MATH_EXACT,
-                  MATH_FAST, MINMOD, NOLIMITER, SLAU, SLAU2, VANALBADA, VANLEER, AusmError, load)
+                  MATH_FAST, MINMOD, NOLIMITER, PREC_FP32, PREC_FP64, SLAU, SLAU2, VANALBADA,
+                  VANLEER, AusmError, load)
 from .batch import Batch
 from .solver import (Solver, circle_boundary, freestream, naca4_boundary, naca4_digits,
                      rect_boundary)
@@ -9,4 +10,4 @@ from .solver import (Solver, circle_boundary, freestream, naca4_boundary, naca4_
 __all__ = ["Solver", "Batch", "AusmError", "load", "naca4_boundary", "naca4_digits", "circle_boundary",
            "rect_boundary", "freestream", "SLAU2", "SLAU", "AUSMUP", "VANALBADA", "MINMOD",
            "VANLEER", "NOLIMITER", "MATH_EXACT", "MATH_FAST", "INIT_REST", "INIT_FREESTREAM",
-           "GRID_JACOBI", "GRID_SOR"]
+           "GRID_JACOBI", "GRID_SOR", "PREC_FP32", "PREC_FP64"]

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libausm.so")
 SLAU2, SLAU, AUSMUP = 0, 1, 2
 VANALBADA, MINMOD, VANLEER, NOLIMITER = 0, 1, 2, 3
 MATH_EXACT, MATH_FAST = 0, 1
+PREC_FP32, PREC_FP64 = 0, 1
 INIT_REST, INIT_FREESTREAM = 0, 1
 GRID_JACOBI, GRID_SOR = 0, 1
 SPACING_POWER, SPACING_COSINE = 0, 1
@@ -73,12 +74,14 @@ SIGNATURES = {
     "ausm_grid_set_vertices": (_i, [_vp, _vp]),
     "ausm_grid_get_vertices": (_i, [_vp, _vp]),
     "ausm_grid_get_vertices64": (_i, [_vp, _vp]),
+    "ausm_grid_set_vertices64": (_i, [_vp, _vp]),
     "ausm_grid_get_metrics": (_i, [_vp, _vp, _vp]),
     "ausm_set_freestream": (_i, [_vp, _d, _d, _d, _d, _d]),
     "ausm_set_scheme": (_i, [_vp, _i, _i, _f, _f]),
     "ausm_set_cfl": (_i, [_vp, _f, _f, _i]),
     "ausm_set_math": (_i, [_vp, _i]),
     "ausm_set_tile": (_i, [_vp, _i]),
+    "ausm_set_precision": (_i, [_vp, _i]),
     "ausm_init_state": (_i, [_vp, _i]),
     "ausm_set_state": (_i, [_vp, _vp]),
     "ausm_get_state": (_i, [_vp, _vp]),
@@ -87,6 +90,9 @@ SIGNATURES = {
     "ausm_wave_speeds": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
     "ausm_forces": (_i, [_vp, _d, _vp, _vp]),
     "ausm_residual_norm": (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "ausm_set_state64": (_i, [_vp, _vp]),
+    "ausm_get_state64": (_i, [_vp, _vp]),
+    "ausm_eval_residual64": (_i, [_vp, _vp]),
     "ausm_profile_stages": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float)]),
     "ausm_batch_create": (_i, [_i, _i, ctypes.c_size_t, _pp]),
     "ausm_batch_destroy": (_i, [_vp]),

@@ -111,8 +111,14 @@ class Solver:
                                               meth, omega))
 
     def set_vertices(self, xy):
-        a = f32(xy).reshape(self.N + 1, self.M, 2)
-        check(self._L.ausm_grid_set_vertices(self.h, ptr(a)))
+        """Vertex upload; float64 input keeps full precision for the FP64
+        solve (the FP32 solve always uses the vertices rounded to FP32)."""
+        if np.asarray(xy).dtype == np.float64:
+            a = f64(xy).reshape(self.N + 1, self.M, 2)
+            check(self._L.ausm_grid_set_vertices64(self.h, ptr(a)))
+        else:
+            a = f32(xy).reshape(self.N + 1, self.M, 2)
+            check(self._L.ausm_grid_set_vertices(self.h, ptr(a)))
 
     def vertices(self, fp64=False):
         out = np.zeros((self.N + 1, self.M, 2), dtype=np.float64 if fp64 else np.float32)
@@ -158,6 +164,18 @@ class Solver:
     def set_tile(self, rows):
         check(self._L.ausm_set_tile(self.h, rows))
 
+    def set_precision(self, prec):
+        """"fp32" (the reference's arithmetic) or "fp64" (the comparison
+        solve); the current solution is carried over."""
+        if isinstance(prec, str):
+            prec = {"fp32": L.PREC_FP32, "fp64": L.PREC_FP64}[prec]
+        check(self._L.ausm_set_precision(self.h, prec))
+        self._prec = prec
+
+    @property
+    def fp64(self):
+        return getattr(self, "_prec", L.PREC_FP32) == L.PREC_FP64
+
     # -- state and stepping --------------------------------------------------
     def prepare_state(self, freestream=True):
         """Fill the state (main.js:631-655); the reference starts from rest,
@@ -165,12 +183,22 @@ class Solver:
         check(self._L.ausm_init_state(self.h, L.INIT_FREESTREAM if freestream else L.INIT_REST))
 
     def set_state(self, q):
-        a = f32(q).reshape(self.N + 3, self.M, 4)
-        check(self._L.ausm_set_state(self.h, ptr(a)))
+        if self.fp64:
+            a = f64(q).reshape(self.N + 3, self.M, 4)
+            check(self._L.ausm_set_state64(self.h, ptr(a)))
+        else:
+            a = f32(q).reshape(self.N + 3, self.M, 4)
+            check(self._L.ausm_set_state(self.h, ptr(a)))
 
     def state(self):
-        out = np.zeros((self.N + 3, self.M, 4), dtype=np.float32)
-        check(self._L.ausm_get_state(self.h, ptr(out)))
+        """(N+3) x M x 4 in the reference's texture layout; float64 in the
+        FP64 precision."""
+        if self.fp64:
+            out = np.zeros((self.N + 3, self.M, 4), dtype=np.float64)
+            check(self._L.ausm_get_state64(self.h, ptr(out)))
+        else:
+            out = np.zeros((self.N + 3, self.M, 4), dtype=np.float32)
+            check(self._L.ausm_get_state(self.h, ptr(out)))
         return out
 
     def step(self, n=1, info=True):
@@ -184,8 +212,12 @@ class Solver:
                 "nonfinite": bool(si.nonfinite), "steps_done": si.steps_done}
 
     def residual(self):
-        out = np.zeros((self.N, self.M, 4), dtype=np.float32)
-        check(self._L.ausm_eval_residual(self.h, ptr(out)))
+        if self.fp64:
+            out = np.zeros((self.N, self.M, 4), dtype=np.float64)
+            check(self._L.ausm_eval_residual64(self.h, ptr(out)))
+        else:
+            out = np.zeros((self.N, self.M, 4), dtype=np.float32)
+            check(self._L.ausm_eval_residual(self.h, ptr(out)))
         return out
 
     def wave_speeds(self):

@@ -66,6 +66,83 @@ void make_freestream(DevParams& p, double rho, double pr, double vx, double vy, 
   fill_derived(p);
 }
 
+// FP64 parameter block: the free stream unrounded; the solver settings the
+// C ABI takes as float widened from the FP32 block.
+void make_freestream64(DevParamsT<double>& p, double rho, double pr, double vx, double vy,
+                       double gamma) {
+  p.gamma = gamma;
+  p.inRho = rho;
+  p.inP = pr;
+  p.vx = vx;
+  p.vy = vy;
+  p.inState[0] = rho;
+  p.inState[1] = vx * rho;
+  p.inState[2] = vy * rho;
+  p.inState[3] = pr / (gamma - 1.0) + 0.5 * (vx * vx + vy * vy) * rho;
+  p.gm1 = gamma - 1.0;
+  p.ggm1 = gamma / (gamma - 1.0);
+  p.minf_coef = (vx * vx + vy * vy) / (gamma * pr);
+  p.astar_coef = 2.0 * (gamma - 1.0) / (gamma + 1.0);
+}
+
+static void widen_settings(ausm_ctx* c) {
+  c->hpar64.maxdt = c->hpar.maxdt;
+  c->hpar64.cfl = c->hpar.cfl;
+  c->hpar64.Kp = c->hpar.Kp;
+  c->hpar64.Ku = c->hpar.Ku;
+}
+
+static bool fp64(const ausm_ctx* c) { return c->prec == AUSM_PREC_FP64; }
+static const double* gx64(const ausm_ctx* c) { return c->g64[c->g_cur]; }
+static const double* gy64(const ausm_ctx* c) { return c->g64[c->g_cur + 1]; }
+
+StageArgs64 stage_args64(ausm_ctx* c, int ty, int stage) {
+  StageArgs64 a{};
+  a.vx = gx64(c);
+  a.vy = gy64(c);
+  a.par = c->dpar64;
+  a.ws_in = c->ws_part64;
+  a.ws_out = c->ws_part64;
+  a.res_out = c->res_part;
+  a.maxws_cur = c->scal_d + 6;
+  a.n_ws_in = c->n_ws_valid;
+  a.M = c->M;
+  a.N = c->N;
+  a.P = c->P;
+  a.PS = c->PS;
+  a.stage = stage;
+  a.limiter = c->limiter;
+  a.tiles_x = stage_tiles_x(c->M);
+  a.qin = c->q64[0];
+  a.qn = c->q64[0];
+  a.qout = c->q64[1];
+  return a;
+}
+
+// One stage launch in the context's precision; in, qn and out index the
+// three state buffers.  `norm`: stage 1 writes the residual-norm partials.
+static hipError_t stage_launch(ausm_ctx* c, int ty, int stage, int in, int qn, int out, bool fresh,
+                               bool refresh, bool norm) {
+  if (fp64(c)) {
+    StageArgs64 a = stage_args64(c, ty, stage);
+    a.qin = c->q64[in];
+    a.qn = c->q64[qn];
+    a.qout = c->q64[out];
+    a.fresh = fresh;
+    a.refresh = refresh;
+    if (!norm) a.res_out = nullptr;
+    return launch_stage64(a, c->scheme, ty, c->stream);
+  }
+  StageArgs a = stage_args(c, ty, stage);
+  a.qin = c->q[in];
+  a.qn = c->q[qn];
+  a.qout = c->q[out];
+  a.fresh = fresh;
+  a.refresh = refresh;
+  if (!norm) a.res_out = nullptr;
+  return launch_stage(a, c->scheme, c->math, ty, c->stream);
+}
+
 // ---- tile autotuner -------------------------------------------------------
 // Rows per wavefront tile of the stage kernel, chosen per (M, N, scheme,
 // math) by timing one stage on the context's own scratch buffer (Q2, which
@@ -76,7 +153,7 @@ static std::map<std::tuple<int, int, int, int, int>, int> g_tune;
 static const int kTileChoices[] = {1, 2, 4, 8};
 
 int autotune_rows(ausm_ctx* c) {
-  const auto key = std::make_tuple(c->M, c->N, c->scheme, c->math, c->limiter);
+  const auto key = std::make_tuple(c->M, c->N, c->scheme, fp64(c) ? 2 : c->math, c->limiter);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
@@ -91,11 +168,10 @@ int autotune_rows(ausm_ctx* c) {
   for (int round = 0; round < 3; ++round) {
     for (int k = 0; k < 4; ++k) {
       const int ty = kTileChoices[k];
-      StageArgs a = stage_args(c, ty, 2);
-      a.qout = c->q[2];
-      for (int w = 0; w < 3; ++w) (void)launch_stage(a, c->scheme, c->math, ty, c->stream);
+      if (fp64(c) && ty > 4) continue;  // FP64 tiles: 1, 2, 4 rows
+      for (int w = 0; w < 3; ++w) (void)stage_launch(c, ty, 2, 0, 0, 2, false, false, false);
       (void)hipEventRecord(e0, c->stream);
-      for (int r = 0; r < 30; ++r) (void)launch_stage(a, c->scheme, c->math, ty, c->stream);
+      for (int r = 0; r < 30; ++r) (void)stage_launch(c, ty, 2, 0, 0, 2, false, false, false);
       (void)hipEventRecord(e1, c->stream);
       (void)hipEventSynchronize(e1);
       float ms = 0.0f;
@@ -146,25 +222,11 @@ StageArgs stage_args(ausm_ctx* c, int ty, int stage) {
 // passes, main.js:744-777).  `norm`: stage 1 also writes the partial sums of
 // the density-residual norm (only the last step of a call needs them).
 hipError_t enqueue_step(ausm_ctx* c, int ty, bool fresh, bool refresh, bool norm) {
-  StageArgs a = stage_args(c, ty, 1);
-  a.fresh = fresh;
-  if (!norm) a.res_out = nullptr;
-  a.qin = c->q[0];
-  a.qout = c->q[1];
-  hipError_t e = launch_stage(a, c->scheme, c->math, ty, c->stream);
+  hipError_t e = stage_launch(c, ty, 1, 0, 0, 1, fresh, false, norm);
   if (e != hipSuccess) return e;
-  a = stage_args(c, ty, 2);
-  a.qin = c->q[1];
-  a.qn = c->q[0];
-  a.qout = c->q[2];
-  e = launch_stage(a, c->scheme, c->math, ty, c->stream);
+  e = stage_launch(c, ty, 2, 1, 0, 2, false, false, false);
   if (e != hipSuccess) return e;
-  a = stage_args(c, ty, 3);
-  a.qin = c->q[2];
-  a.qn = c->q[0];
-  a.qout = c->q[0];
-  a.refresh = refresh;
-  return launch_stage(a, c->scheme, c->math, ty, c->stream);
+  return stage_launch(c, ty, 3, 2, 0, 0, false, refresh, false);
 }
 
 void drop_graphs(ausm_ctx* c) {
@@ -221,6 +283,7 @@ int ausm_ctx_create(int device, int M, int N, ausm_ctx** out) {
   ok = ok && hipMalloc(&c->scal_f, sizeof(float) * 8) == hipSuccess;
   ok = ok && hipMalloc(&c->scal_d, sizeof(double) * 8) == hipSuccess;
   ok = ok && hipMalloc(&c->dpar, sizeof(DevParams)) == hipSuccess;
+  ok = ok && hipMalloc(&c->dpar64, sizeof(DevParamsT<double>)) == hipSuccess;
   ok = ok && hipMalloc(&c->stage_buf, sizeof(float) * 4 * (size_t)M * (N + 3)) == hipSuccess;
   ok = ok && hipMalloc(&c->seg, sizeof(double) * (2 * (size_t)M + M)) == hipSuccess;
   ok = ok && hipHostMalloc(&c->pinned, 256) == hipSuccess;
@@ -230,10 +293,12 @@ int ausm_ctx_create(int device, int M, int N, ausm_ctx** out) {
   }
   // defaults: the reference's start-up uniforms (main.js:874-887, util.js:76-99)
   make_freestream(c->hpar, 1.0, 1.0 / 1.4, 0.0, 0.0, 1.4);
+  make_freestream64(c->hpar64, 1.0, 1.0 / 1.4, 0.0, 0.0, 1.4);
   c->hpar.maxdt = 1e-4f;
   c->hpar.cfl = 1.5f;
   c->hpar.Kp = 0.0f;
   c->hpar.Ku = 0.75f;
+  widen_settings(c);
   c->scheme = SCHEME_SLAU2;
   c->limiter = LIM_VANALBADA;
   c->math = 0;
@@ -241,7 +306,10 @@ int ausm_ctx_create(int device, int M, int N, ausm_ctx** out) {
   c->tile = 0;
   c->use_graphs = getenv("AUSM_NO_GRAPHS") == nullptr;
   (void)hipMemcpyAsync(c->dpar, &c->hpar, sizeof(DevParams), hipMemcpyHostToDevice, c->stream);
+  (void)hipMemcpyAsync(c->dpar64, &c->hpar64, sizeof(DevParamsT<double>), hipMemcpyHostToDevice,
+                       c->stream);
   (void)hipMemsetAsync(c->scal_f, 0, sizeof(float) * 8, c->stream);
+  (void)hipMemsetAsync(c->scal_d, 0, sizeof(double) * 8, c->stream);
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     ausm_ctx_destroy(c);
     return set_error(AUSM_ERR_HIP, "context initialisation failed");
@@ -264,6 +332,10 @@ int ausm_ctx_destroy(ausm_ctx* c) {
   hipFree(c->scal_f);
   hipFree(c->scal_d);
   hipFree(c->dpar);
+  hipFree(c->dpar64);
+  for (int k = 0; k < 3; ++k) hipFree(c->q64[k]);
+  hipFree(c->ws_part64);
+  hipFree(c->stage_buf64);
   hipFree(c->stage_buf);
   hipFree(c->seg);
   if (c->pinned) hipHostFree(c->pinned);
@@ -423,10 +495,33 @@ int ausm_grid_set_vertices(ausm_ctx* c, const float* xy) {
     }
   CHECK_HIP(hipMemcpyAsync(c->vx, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
   CHECK_HIP(hipMemcpyAsync(c->vy, hy.data(), sizeof(float) * hy.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
+  // the FP64 solve uses the same vertices, widened
+  std::vector<double> dx(hx.begin(), hx.end()), dy(hy.begin(), hy.end());
+  CHECK_HIP(hipMemcpyAsync(c->g64[0], dx.data(), sizeof(double) * dx.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
+  CHECK_HIP(hipMemcpyAsync(c->g64[1], dy.data(), sizeof(double) * dy.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
   CHECK_HIP(hipStreamSynchronize(c->stream), "vertex upload");
   c->has_grid = true;
   c->has_state = false;
-  c->g_cur = -1;
+  c->g_cur = 0;
+  return AUSM_OK;
+}
+
+int ausm_grid_set_vertices64(ausm_ctx* c, const double* xy) {
+  if (!c || !xy) return set_error(AUSM_ERR_ARG, "NULL argument");
+  hipSetDevice(c->device);
+  const int M = c->M, N = c->N, P = c->P;
+  std::vector<double> hx((size_t)(N + 1) * P, 0.0), hy((size_t)(N + 1) * P, 0.0);
+  for (int j = 0; j <= N; ++j)
+    for (int i = 0; i < M; ++i) {
+      hx[(size_t)j * P + i] = xy[((size_t)j * M + i) * 2];
+      hy[(size_t)j * P + i] = xy[((size_t)j * M + i) * 2 + 1];
+    }
+  CHECK_HIP(hipMemcpyAsync(c->g64[0], hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
+  CHECK_HIP(hipMemcpyAsync(c->g64[1], hy.data(), sizeof(double) * hy.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
+  c->g_cur = 0;
+  const int r = finish_grid(c, c->g64[0], c->g64[1]);
+  if (r != AUSM_OK) return r;
+  CHECK_HIP(hipStreamSynchronize(c->stream), "vertex upload");
   return AUSM_OK;
 }
 
@@ -449,7 +544,7 @@ int ausm_grid_get_vertices(ausm_ctx* c, float* xy) {
 
 int ausm_grid_get_vertices64(ausm_ctx* c, double* xy) {
   if (!c || !xy) return set_error(AUSM_ERR_ARG, "NULL argument");
-  if (c->g_cur < 0) return set_error(AUSM_ERR_ORDER, "no FP64 grid (vertices were uploaded)");
+  if (c->g_cur < 0) return set_error(AUSM_ERR_ORDER, "no grid");
   hipSetDevice(c->device);
   const int M = c->M, N = c->N, P = c->P;
   std::vector<double> hx((size_t)(N + 1) * P), hy((size_t)(N + 1) * P);
@@ -480,7 +575,9 @@ int ausm_grid_get_metrics(ausm_ctx* c, float* area, float* flen) {
 
 // ---- parameters ---------------------------------------------------------------
 static int upload_params(ausm_ctx* c) {
+  widen_settings(c);
   CHECK_HIP(hipMemcpyAsync(c->dpar, &c->hpar, sizeof(DevParams), hipMemcpyHostToDevice, c->stream), "parameter upload");
+  CHECK_HIP(hipMemcpyAsync(c->dpar64, &c->hpar64, sizeof(DevParamsT<double>), hipMemcpyHostToDevice, c->stream), "parameter upload");
   CHECK_HIP(hipStreamSynchronize(c->stream), "parameter upload");
   return AUSM_OK;
 }
@@ -491,6 +588,7 @@ int ausm_set_freestream(ausm_ctx* c, double rho, double p, double vx, double vy,
     return set_error(AUSM_ERR_ARG, "bad free stream (rho=%g p=%g gamma=%g)", rho, p, gamma);
   hipSetDevice(c->device);
   make_freestream(c->hpar, rho, p, vx, vy, gamma);
+  make_freestream64(c->hpar64, rho, p, vx, vy, gamma);
   return upload_params(c);
 }
 
@@ -541,9 +639,14 @@ int ausm_set_tile(ausm_ctx* c, int rows) {
 static int refresh_ws(ausm_ctx* c) {
   // wave speeds of the current solution -> per-block maxima (the post pass
   // of prepareState, main.js:649-652)
-  CHECK_HIP(launch_wave_speed(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, nullptr,
-                              c->ws_part, c->stream),
-            "wave speeds");
+  if (fp64(c))
+    CHECK_HIP(launch_wave_speed(c->q64[0], gx64(c), gy64(c), c->dpar64, c->M, c->N, c->P, c->PS,
+                                (double*)nullptr, c->ws_part64, c->stream),
+              "wave speeds");
+  else
+    CHECK_HIP(launch_wave_speed(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS,
+                                (float*)nullptr, c->ws_part, c->stream),
+              "wave speeds");
   c->n_ws_valid = wave_speed_blocks(c->M, c->N);
   c->ws_fresh = true;
   return AUSM_OK;
@@ -555,7 +658,10 @@ int ausm_init_state(ausm_ctx* c, int mode) {
   if (mode != AUSM_INIT_REST && mode != AUSM_INIT_FREESTREAM)
     return set_error(AUSM_ERR_ARG, "unknown init mode %d", mode);
   hipSetDevice(c->device);
-  CHECK_HIP(launch_fill_state(c->q[0], c->dpar, c->M, c->N, c->P, c->PS, mode, c->stream), "init state");
+  if (fp64(c))
+    CHECK_HIP(launch_fill_state(c->q64[0], c->dpar64, c->M, c->N, c->P, c->PS, mode, c->stream), "init state");
+  else
+    CHECK_HIP(launch_fill_state(c->q[0], c->dpar, c->M, c->N, c->P, c->PS, mode, c->stream), "init state");
   int r = refresh_ws(c);
   if (r != AUSM_OK) return r;
   CHECK_HIP(hipStreamSynchronize(c->stream), "init state");
@@ -571,7 +677,12 @@ int ausm_set_state(ausm_ctx* c, const float* q) {
   hipSetDevice(c->device);
   const size_t n = 4 * (size_t)c->M * (c->N + 3);
   CHECK_HIP(hipMemcpyAsync(c->stage_buf, q, sizeof(float) * n, hipMemcpyHostToDevice, c->stream), "state upload");
-  CHECK_HIP(launch_aos_to_soa(c->stage_buf, c->q[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+  if (fp64(c)) {
+    CHECK_HIP(launch_aos_to_soa(c->stage_buf, c->q[1], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+    CHECK_HIP(launch_convert_planes(c->q[1], c->q64[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+  } else {
+    CHECK_HIP(launch_aos_to_soa(c->stage_buf, c->q[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+  }
   int r = refresh_ws(c);
   if (r != AUSM_OK) return r;
   CHECK_HIP(hipStreamSynchronize(c->stream), "state upload");
@@ -586,13 +697,92 @@ int ausm_get_state(ausm_ctx* c, float* q) {
   if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
   hipSetDevice(c->device);
   const size_t n = 4 * (size_t)c->M * (c->N + 3);
-  CHECK_HIP(launch_soa_to_aos(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, c->stage_buf, c->stream), "state download");
+  if (fp64(c)) {
+    CHECK_HIP(launch_soa_to_aos(c->q64[0], gx64(c), gy64(c), c->dpar64, c->M, c->N, c->P, c->PS, c->stage_buf64, c->stream), "state download");
+    CHECK_HIP(launch_f64_to_f32(c->stage_buf64, c->stage_buf, n, c->stream), "state download");
+  } else {
+    CHECK_HIP(launch_soa_to_aos(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, c->stage_buf, c->stream), "state download");
+  }
   CHECK_HIP(hipMemcpyAsync(q, c->stage_buf, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "state download");
   CHECK_HIP(hipStreamSynchronize(c->stream), "state download");
   return AUSM_OK;
 }
 
-static int current_rows(ausm_ctx* c) { return c->tile ? c->tile : autotune_rows(c); }
+static int current_rows(ausm_ctx* c) {
+  const int t = c->tile ? c->tile : autotune_rows(c);
+  return fp64(c) ? std::min(t, 4) : t;  // FP64 tiles: 1, 2, 4 rows
+}
+
+int ausm_set_precision(ausm_ctx* c, int precision) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (precision != AUSM_PREC_FP32 && precision != AUSM_PREC_FP64)
+    return set_error(AUSM_ERR_ARG, "unknown precision %d", precision);
+  if (precision == c->prec) return AUSM_OK;
+  hipSetDevice(c->device);
+  if (precision == AUSM_PREC_FP64 && !c->q64[0]) {
+    const size_t nq = sizeof(double) * 4 * (size_t)c->PS;
+    bool ok = true;
+    for (int k = 0; k < 3 && ok; ++k) ok = hipMalloc(&c->q64[k], nq) == hipSuccess;
+    ok = ok && hipMalloc(&c->ws_part64, sizeof(double) * c->max_parts) == hipSuccess;
+    ok = ok && hipMalloc(&c->stage_buf64, sizeof(double) * 4 * (size_t)c->M * (c->N + 3)) == hipSuccess;
+    if (!ok) {
+      for (int k = 0; k < 3; ++k) {
+        hipFree(c->q64[k]);
+        c->q64[k] = nullptr;
+      }
+      hipFree(c->ws_part64);
+      hipFree(c->stage_buf64);
+      c->ws_part64 = nullptr;
+      c->stage_buf64 = nullptr;
+      return set_error(AUSM_ERR_NOMEM, "device allocation failed for the FP64 state");
+    }
+  }
+  drop_graphs(c);
+  // carry the current solution over to the new precision
+  if (c->has_state) {
+    if (precision == AUSM_PREC_FP64)
+      CHECK_HIP(launch_convert_planes(c->q[0], c->q64[0], c->M, c->N, c->P, c->PS, c->stream), "precision switch");
+    else
+      CHECK_HIP(launch_convert_planes(c->q64[0], c->q[0], c->M, c->N, c->P, c->PS, c->stream), "precision switch");
+  }
+  c->prec = precision;
+  if (c->has_state) {
+    const int r = refresh_ws(c);
+    if (r != AUSM_OK) return r;
+    c->since_refresh = 0;
+  }
+  CHECK_HIP(hipStreamSynchronize(c->stream), "precision switch");
+  return AUSM_OK;
+}
+
+int ausm_set_state64(ausm_ctx* c, const double* q) {
+  if (!c || !q) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!fp64(c)) return set_error(AUSM_ERR_ORDER, "FP64 state needs FP64 precision");
+  if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "set a grid before the state");
+  hipSetDevice(c->device);
+  const size_t n = 4 * (size_t)c->M * (c->N + 3);
+  CHECK_HIP(hipMemcpyAsync(c->stage_buf64, q, sizeof(double) * n, hipMemcpyHostToDevice, c->stream), "state upload");
+  CHECK_HIP(launch_aos_to_soa(c->stage_buf64, c->q64[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+  const int r = refresh_ws(c);
+  if (r != AUSM_OK) return r;
+  CHECK_HIP(hipStreamSynchronize(c->stream), "state upload");
+  c->has_state = true;
+  c->steps = 0;
+  c->since_refresh = 0;
+  return AUSM_OK;
+}
+
+int ausm_get_state64(ausm_ctx* c, double* q) {
+  if (!c || !q) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!fp64(c)) return set_error(AUSM_ERR_ORDER, "FP64 state needs FP64 precision");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  const size_t n = 4 * (size_t)c->M * (c->N + 3);
+  CHECK_HIP(launch_soa_to_aos(c->q64[0], gx64(c), gy64(c), c->dpar64, c->M, c->N, c->P, c->PS, c->stage_buf64, c->stream), "state download");
+  CHECK_HIP(hipMemcpyAsync(q, c->stage_buf64, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream), "state download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "state download");
+  return AUSM_OK;
+}
 
 int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
   if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
@@ -655,16 +845,31 @@ int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
     // density-residual norm of the last step's stage 1 and the current max
     // wave speed; one small read-back
     CHECK_HIP(launch_reduce_sum_f64(c->res_part, c->N * stage_tiles_x(c->M), c->scal_d, c->stream), "residual norm");
-    CHECK_HIP(launch_reduce_max(c->ws_part, c->n_ws_valid, c->scal_f + 1, c->stream), "max wave speed");
-    CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "info");
-    CHECK_HIP(hipMemcpyAsync((char*)c->pinned + 8, c->scal_f, 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "info");
-    CHECK_HIP(hipStreamSynchronize(c->stream), "step");
-    const double l2 = *(double*)c->pinned;
-    const float cur = ((float*)((char*)c->pinned + 8))[0];
-    const float nxt = ((float*)((char*)c->pinned + 8))[1];
+    double l2, cur, nxt, dt;
+    if (fp64(c)) {
+      CHECK_HIP(launch_reduce_max(c->ws_part64, c->n_ws_valid, c->scal_d + 7, c->stream), "max wave speed");
+      CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "info");
+      CHECK_HIP(hipMemcpyAsync((char*)c->pinned + 8, c->scal_d + 6, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "info");
+      CHECK_HIP(hipStreamSynchronize(c->stream), "step");
+      const double* h = (const double*)c->pinned;
+      l2 = h[0];
+      cur = h[1];
+      nxt = h[2];
+      dt = std::fmin(c->hpar64.maxdt, c->hpar64.cfl / cur);
+    } else {
+      CHECK_HIP(launch_reduce_max(c->ws_part, c->n_ws_valid, c->scal_f + 1, c->stream), "max wave speed");
+      CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "info");
+      CHECK_HIP(hipMemcpyAsync((char*)c->pinned + 8, c->scal_f, 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "info");
+      CHECK_HIP(hipStreamSynchronize(c->stream), "step");
+      l2 = *(double*)c->pinned;
+      const float* h = (const float*)((char*)c->pinned + 8);
+      cur = h[0];
+      nxt = h[1];
+      dt = std::fmin(c->hpar.maxdt, c->hpar.cfl / h[0]);
+    }
     info->res_l2 = nsteps > 0 ? l2 : -1.0;
-    info->max_wave_speed = cur;
-    info->dt = std::fmin(c->hpar.maxdt, c->hpar.cfl / cur);
+    info->max_wave_speed = (float)cur;
+    info->dt = (float)dt;
     info->nonfinite = !(std::isfinite(cur) && std::isfinite(nxt)) || !std::isfinite(l2);
     info->steps_done = c->steps;
     if (info->nonfinite) return set_error(AUSM_ERR_NONFINITE, "non-finite solution after %lld steps", c->steps);
@@ -675,17 +880,39 @@ int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
   return AUSM_OK;
 }
 
+// stage 0 writes L(Q^n) into the scratch buffer Q2 (every step rewrites it
+// before reading it)
+static int residual_to_staging(ausm_ctx* c) {
+  const int ty = current_rows(c);
+  CHECK_HIP(stage_launch(c, ty, 0, 0, 0, 2, false, false, false), "residual");
+  if (fp64(c))
+    CHECK_HIP(launch_planes_to_aos(c->q64[2], c->M, c->N, c->P, c->PS, c->stage_buf64, c->stream), "residual");
+  else
+    CHECK_HIP(launch_planes_to_aos(c->q[2], c->M, c->N, c->P, c->PS, c->stage_buf, c->stream), "residual");
+  return AUSM_OK;
+}
+
 int ausm_eval_residual(ausm_ctx* c, float* res) {
   if (!c || !res) return set_error(AUSM_ERR_ARG, "NULL argument");
   if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
   hipSetDevice(c->device);
-  const int ty = current_rows(c);
-  StageArgs a = stage_args(c, ty, 0);
-  a.qin = c->q[0];
-  a.qout = c->q[2];  // scratch: every step rewrites Q2 before reading it
-  CHECK_HIP(launch_stage(a, c->scheme, c->math, ty, c->stream), "residual");
-  CHECK_HIP(launch_planes_to_aos(c->q[2], c->M, c->N, c->P, c->PS, c->stage_buf, c->stream), "residual");
-  CHECK_HIP(hipMemcpyAsync(res, c->stage_buf, sizeof(float) * 4 * (size_t)c->M * c->N, hipMemcpyDeviceToHost, c->stream), "residual download");
+  const size_t n = 4 * (size_t)c->M * c->N;
+  const int r = residual_to_staging(c);
+  if (r != AUSM_OK) return r;
+  if (fp64(c)) CHECK_HIP(launch_f64_to_f32(c->stage_buf64, c->stage_buf, n, c->stream), "residual");
+  CHECK_HIP(hipMemcpyAsync(res, c->stage_buf, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "residual download");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "residual");
+  return AUSM_OK;
+}
+
+int ausm_eval_residual64(ausm_ctx* c, double* res) {
+  if (!c || !res) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (!fp64(c)) return set_error(AUSM_ERR_ORDER, "FP64 residual needs FP64 precision");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  const int r = residual_to_staging(c);
+  if (r != AUSM_OK) return r;
+  CHECK_HIP(hipMemcpyAsync(res, c->stage_buf64, sizeof(double) * 4 * (size_t)c->M * c->N, hipMemcpyDeviceToHost, c->stream), "residual download");
   CHECK_HIP(hipStreamSynchronize(c->stream), "residual");
   return AUSM_OK;
 }
@@ -694,12 +921,23 @@ int ausm_wave_speeds(ausm_ctx* c, float* ws, float* max_ws) {
   if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
   if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
   hipSetDevice(c->device);
+  const size_t mn = (size_t)c->M * c->N;
   float* dws = c->stage_buf;
-  float* part = c->stage_buf + (size_t)c->M * c->N;
   const int nb = wave_speed_blocks(c->M, c->N);
-  CHECK_HIP(launch_wave_speed(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, dws, part, c->stream), "wave speeds");
-  CHECK_HIP(launch_reduce_max(part, nb, c->scal_f + 2, c->stream), "max wave speed");
-  if (ws) CHECK_HIP(hipMemcpyAsync(ws, dws, sizeof(float) * (size_t)c->M * c->N, hipMemcpyDeviceToHost, c->stream), "wave speed download");
+  if (fp64(c)) {
+    // FP64 speeds and maximum, returned rounded to FP32
+    double* ws64 = c->stage_buf64;
+    double* part = c->stage_buf64 + mn;
+    CHECK_HIP(launch_wave_speed(c->q64[0], gx64(c), gy64(c), c->dpar64, c->M, c->N, c->P, c->PS, ws64, part, c->stream), "wave speeds");
+    CHECK_HIP(launch_reduce_max(part, nb, c->scal_d + 7, c->stream), "max wave speed");
+    CHECK_HIP(launch_f64_to_f32(ws64, dws, mn, c->stream), "wave speeds");
+    CHECK_HIP(launch_f64_to_f32(c->scal_d + 7, c->scal_f + 2, 1, c->stream), "wave speeds");
+  } else {
+    float* part = c->stage_buf + mn;
+    CHECK_HIP(launch_wave_speed(c->q[0], c->vx, c->vy, c->dpar, c->M, c->N, c->P, c->PS, dws, part, c->stream), "wave speeds");
+    CHECK_HIP(launch_reduce_max(part, nb, c->scal_f + 2, c->stream), "max wave speed");
+  }
+  if (ws) CHECK_HIP(hipMemcpyAsync(ws, dws, sizeof(float) * mn, hipMemcpyDeviceToHost, c->stream), "wave speed download");
   CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_f + 2, sizeof(float), hipMemcpyDeviceToHost, c->stream), "wave speed download");
   CHECK_HIP(hipStreamSynchronize(c->stream), "wave speeds");
   if (max_ws) *max_ws = *(float*)c->pinned;
@@ -712,7 +950,10 @@ int ausm_forces(ausm_ctx* c, double ref_length, double* out, double* cp) {
   if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
   hipSetDevice(c->device);
   double* dcp = c->seg + 2 * (size_t)c->M;
-  CHECK_HIP(launch_forces(c->q[0], c->vx, c->vy, c->dpar, c->M, c->PS, ref_length, c->seg, c->scal_d + 2, dcp, c->stream), "forces");
+  if (fp64(c))
+    CHECK_HIP(launch_forces(c->q64[0], gx64(c), gy64(c), c->dpar64, c->M, c->PS, ref_length, c->seg, c->scal_d + 2, dcp, c->stream), "forces");
+  else
+    CHECK_HIP(launch_forces(c->q[0], c->vx, c->vy, c->dpar, c->M, c->PS, ref_length, c->seg, c->scal_d + 2, dcp, c->stream), "forces");
   CHECK_HIP(hipMemcpyAsync(out, c->scal_d + 2, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "forces download");
   if (cp) CHECK_HIP(hipMemcpyAsync(cp, dcp, sizeof(double) * c->M, hipMemcpyDeviceToHost, c->stream), "cp download");
   CHECK_HIP(hipStreamSynchronize(c->stream), "forces");
@@ -726,11 +967,7 @@ int ausm_residual_norm(ausm_ctx* c, double* l2) {
   // a stage-1 launch into the scratch buffer Q1 accumulates the per-block
   // density-residual sums of the current solution
   const int ty = current_rows(c);
-  StageArgs a = stage_args(c, ty, 1);
-  a.qin = c->q[0];
-  a.qout = c->q[1];
-  a.fresh = 0;
-  CHECK_HIP(launch_stage(a, c->scheme, c->math, ty, c->stream), "residual norm");
+  CHECK_HIP(stage_launch(c, ty, 1, 0, 0, 1, false, false, true), "residual norm");
   CHECK_HIP(launch_reduce_sum_f64(c->res_part, c->N * stage_tiles_x(c->M), c->scal_d, c->stream), "residual norm");
   CHECK_HIP(hipMemcpyAsync(c->pinned, c->scal_d, sizeof(double), hipMemcpyDeviceToHost, c->stream), "residual norm");
   CHECK_HIP(hipStreamSynchronize(c->stream), "residual norm");
@@ -747,17 +984,13 @@ int ausm_profile_stages(ausm_ctx* c, int reps, float* ms3) {
   CHECK_HIP(hipEventCreate(&e0), "event");
   CHECK_HIP(hipEventCreate(&e1), "event");
   for (int st = 1; st <= 3; ++st) {
-    StageArgs a = stage_args(c, ty, st);
-    a.qin = st == 1 ? c->q[0] : (st == 2 ? c->q[1] : c->q[2]);
-    a.qn = c->q[0];
-    a.qout = st == 2 ? c->q[2] : c->q[1];  // scratch outputs only
-    a.fresh = 0;
-    a.refresh = st == 3;
-    a.res_out = nullptr;  // the residual norm is formed on check steps only
+    // scratch outputs only; the residual norm is formed on check steps only
+    const int in = st == 1 ? 0 : (st == 2 ? 1 : 2);
+    const int out = st == 2 ? 2 : 1;
     double total = 0.0;
     for (int k = 0; k < reps; ++k) {
       CHECK_HIP(hipEventRecord(e0, c->stream), "event");
-      CHECK_HIP(launch_stage(a, c->scheme, c->math, ty, c->stream), "stage");
+      CHECK_HIP(stage_launch(c, ty, st, in, 0, out, false, st == 3, false), "stage");
       CHECK_HIP(hipEventRecord(e1, c->stream), "event");
       CHECK_HIP(hipEventSynchronize(e1), "event");
       float ms = 0.0f;

@@ -124,39 +124,40 @@ __global__ void metrics_kernel(const float* vx, const float* vy, int M, int N, i
 // --------------------------------------------------------------------------
 // wave speeds
 // --------------------------------------------------------------------------
-__device__ __forceinline__ float nan_max2(float m, float x) { return (x > m || x != x) ? x : m; }
+template <typename R>
+__device__ __forceinline__ R nan_max2(R m, R x) { return (x > m || x != x) ? x : m; }
 
-__global__ void wave_speed_kernel(const float* q, const float* vx, const float* vy,
-                                  const DevParams* par, int M, int N, int P, int PS, float* ws,
-                                  float* part) {
+template <typename R>
+__global__ void wave_speed_kernel(const R* q, const R* vx, const R* vy, const DevParamsT<R>* par,
+                                  int M, int N, int P, int PS, R* ws, R* part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  float w = 0.0f;
+  R w = R(0);
   if (i < M * N) {
     const int y = i / M, x = i - y * M;
     const int r = (x + 1 == M) ? 0 : x + 1;
-    const float ax = vx[y * P + x], ay = vy[y * P + x];
-    const float bx = vx[y * P + r], by = vy[y * P + r];
-    const float cx = vx[(y + 1) * P + x], cy = vy[(y + 1) * P + x];
-    const float dx = vx[(y + 1) * P + r], dy = vy[(y + 1) * P + r];
-    const float area = 0.5f * fabsf((dx - ax) * (cy - by) - (cx - bx) * (dy - ay));
-    float ex, ey, fl[4];
-    ex = ax - cx; ey = ay - cy; fl[0] = sqrtf(ex * ex + ey * ey);
-    ex = bx - dx; ey = by - dy; fl[1] = sqrtf(ex * ex + ey * ey);
-    ex = cx - dx; ey = cy - dy; fl[2] = sqrtf(ex * ex + ey * ey);
-    ex = ax - bx; ey = ay - by; fl[3] = sqrtf(ex * ex + ey * ey);
+    const R ax = vx[y * P + x], ay = vy[y * P + x];
+    const R bx = vx[y * P + r], by = vy[y * P + r];
+    const R cx = vx[(y + 1) * P + x], cy = vy[(y + 1) * P + x];
+    const R dx = vx[(y + 1) * P + r], dy = vy[(y + 1) * P + r];
+    const R area = R(0.5) * fabs((dx - ax) * (cy - by) - (cx - bx) * (dy - ay));
+    R ex, ey, fl[4];
+    ex = ax - cx; ey = ay - cy; fl[0] = sqrt(ex * ex + ey * ey);
+    ex = bx - dx; ey = by - dy; fl[1] = sqrt(ex * ex + ey * ey);
+    ex = cx - dx; ey = cy - dy; fl[2] = sqrt(ex * ex + ey * ey);
+    ex = ax - bx; ey = ay - by; fl[3] = sqrt(ex * ex + ey * ey);
     const int o = y * P + x;
-    const float rho = q[o];
-    const float u = q[PS + o] / rho, v = q[2 * PS + o] / rho;
-    const float vv = u * u + v * v;
-    const float p = (q[3 * PS + o] - (0.5f * rho) * vv) * (par->gamma - 1.0f);
-    const float T = p / rho;
-    const float c = sqrtf(par->gamma * T);
-    const float speed = sqrtf(vv);
-    const float S = ((fl[0] + fl[1]) + fl[2]) + fl[3];
-    w = ((speed + c) * S) / (2.0f * area);
+    const R rho = q[o];
+    const R u = q[PS + o] / rho, v = q[2 * PS + o] / rho;
+    const R vv = u * u + v * v;
+    const R p = (q[3 * PS + o] - (R(0.5) * rho) * vv) * (par->gamma - R(1));
+    const R T = p / rho;
+    const R c = sqrt(par->gamma * T);
+    const R speed = sqrt(vv);
+    const R S = ((fl[0] + fl[1]) + fl[2]) + fl[3];
+    w = ((speed + c) * S) / (R(2) * area);
     if (ws) ws[i] = w;
   }
-  __shared__ float sm[256];
+  __shared__ R sm[256];
   sm[threadIdx.x] = w;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -166,10 +167,11 @@ __global__ void wave_speed_kernel(const float* q, const float* vx, const float* 
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-__global__ void reduce_max_kernel(const float* part, int n, float* out) {
-  float m = 0.0f;
+template <typename R>
+__global__ void reduce_max_kernel(const R* part, int n, R* out) {
+  R m = R(0);
   for (int i = threadIdx.x; i < n; i += blockDim.x) m = nan_max2(m, part[i]);
-  __shared__ float sm[256];
+  __shared__ R sm[256];
   sm[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -198,20 +200,20 @@ __global__ void reduce_sum_f64_kernel(const double* part, int n, double* out) {
 // forces (graphs.js:16-101): per-segment p*l*(-n) from row-1 pressure,
 // summed in FP64 in index order
 // --------------------------------------------------------------------------
-__global__ void forces_kernel(const float* q, const float* vx, const float* vy,
-                              const DevParams* par, int M, int PS, double ref_len, double* seg,
-                              double* out, double* cp) {
+template <typename R>
+__global__ void forces_kernel(const R* q, const R* vx, const R* vy, const DevParamsT<R>* par, int M,
+                              int PS, double ref_len, double* seg, double* out, double* cp) {
   const double Vx = par->vx, Vy = par->vy;
   const double qinf = 0.5 * (double)par->inRho * (Vx * Vx + Vy * Vy);
   for (int i = threadIdx.x; i < M; i += blockDim.x) {
-    const float rho = q[i];
-    const float u = q[PS + i] / rho, v = q[2 * PS + i] / rho;
-    const float p = (q[3 * PS + i] - (0.5f * rho) * (u * u + v * v)) * (par->gamma - 1.0f);
+    const R rho = q[i];
+    const R u = q[PS + i] / rho, v = q[2 * PS + i] / rho;
+    const R p = (q[3 * PS + i] - (R(0.5) * rho) * (u * u + v * v)) * (par->gamma - R(1));
     const int r = (i + 1 == M) ? 0 : i + 1;
-    const float ex = vx[r] - vx[i], ey = vy[r] - vy[i];
-    const float fl = sqrtf(ex * ex + ey * ey);
-    const float nx = (vy[r] - vy[i]) / fl, ny = (vx[i] - vx[r]) / fl;
-    const float pf = p * fl;
+    const R ex = vx[r] - vx[i], ey = vy[r] - vy[i];
+    const R fl = sqrt(ex * ex + ey * ey);
+    const R nx = (vy[r] - vy[i]) / fl, ny = (vx[i] - vx[r]) / fl;
+    const R pf = p * fl;
     seg[2 * i] = (double)(pf * -nx);
     seg[2 * i + 1] = (double)(pf * -ny);
     if (cp) cp[i] = ((double)p - (double)par->inP) / qinf;
@@ -236,11 +238,12 @@ __global__ void forces_kernel(const float* q, const float* vx, const float* vy,
 // layouts
 // --------------------------------------------------------------------------
 // reference layout (N+3) x M x 4, rows 1..N -> SoA planes
-__global__ void aos_to_soa_kernel(const float* aos, float* q, int M, int N, int P, int PS) {
+template <typename R>
+__global__ void aos_to_soa_kernel(const R* aos, R* q, int M, int N, int P, int PS) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= M || y >= N) return;
-  const float* s = aos + ((size_t)(y + 1) * M + x) * 4;
+  const R* s = aos + ((size_t)(y + 1) * M + x) * 4;
   const int o = y * P + x;
   q[o] = s[0];
   q[PS + o] = s[1];
@@ -249,29 +252,30 @@ __global__ void aos_to_soa_kernel(const float* aos, float* q, int M, int N, int 
 }
 
 // SoA planes -> reference layout, ghost rows as the boundary pass makes them
-__global__ void soa_to_aos_kernel(const float* q, const float* vx, const float* vy,
-                                  const DevParams* par, int M, int N, int P, int PS, float* aos) {
+template <typename R>
+__global__ void soa_to_aos_kernel(const R* q, const R* vx, const R* vy, const DevParamsT<R>* par,
+                                  int M, int N, int P, int PS, R* aos) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;  // 0 .. N+2
   if (x >= M || y > N + 2) return;
   const int r = (x + 1 == M) ? 0 : x + 1;
-  Cons g;
+  ConsT<R> g;
   auto ld = [&](int j) {
     const int o = j * P + x;
-    return Cons{q[o], q[PS + o], q[2 * PS + o], q[3 * PS + o]};
+    return ConsT<R>{q[o], q[PS + o], q[2 * PS + o], q[3 * PS + o]};
   };
   if (y >= 1 && y <= N) {
     g = ld(y - 1);
   } else if (y == 0) {
-    float nx, ny, len;
+    R nx, ny, len;
     normal_exact(vx[x] - vx[r], vy[x] - vy[r], nx, ny, len);
     g = wall_ghost(ld(0), nx, ny);
   } else {
-    float nx, ny, len;
+    R nx, ny, len;
     normal_exact(vx[N * P + x] - vx[N * P + r], vy[N * P + x] - vy[N * P + r], nx, ny, len);
     g = far_ghost(*par, ld(N - 1), nx, ny, y == N + 2);
   }
-  float* d = aos + ((size_t)y * M + x) * 4;
+  R* d = aos + ((size_t)y * M + x) * 4;
   d[0] = g.r;
   d[1] = g.mu;
   d[2] = g.mv;
@@ -279,12 +283,13 @@ __global__ void soa_to_aos_kernel(const float* q, const float* vx, const float* 
 }
 
 // residual planes -> N x M x 4
-__global__ void planes_to_aos_kernel(const float* q, int M, int N, int P, int PS, float* aos) {
+template <typename R>
+__global__ void planes_to_aos_kernel(const R* q, int M, int N, int P, int PS, R* aos) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= M || y >= N) return;
   const int o = y * P + x;
-  float* d = aos + ((size_t)y * M + x) * 4;
+  R* d = aos + ((size_t)y * M + x) * 4;
   d[0] = q[o];
   d[1] = q[PS + o];
   d[2] = q[2 * PS + o];
@@ -292,7 +297,8 @@ __global__ void planes_to_aos_kernel(const float* q, int M, int N, int P, int PS
 }
 
 // uniform state: rest (ausm.js:14-16) or inState
-__global__ void fill_state_kernel(float* q, const DevParams* par, int M, int N, int P, int PS,
+template <typename R>
+__global__ void fill_state_kernel(R* q, const DevParamsT<R>* par, int M, int N, int P, int PS,
                                   int freestream) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
@@ -305,10 +311,20 @@ __global__ void fill_state_kernel(float* q, const DevParams* par, int M, int N, 
     q[3 * PS + o] = par->inState[3];
   } else {
     q[o] = par->inRho;
-    q[PS + o] = 0.0f;
-    q[2 * PS + o] = 0.0f;
-    q[3 * PS + o] = par->inP / (par->gamma - 1.0f);
+    q[PS + o] = R(0);
+    q[2 * PS + o] = R(0);
+    q[3 * PS + o] = par->inP / (par->gamma - R(1));
   }
+}
+
+// state planes between precisions (interior rows, 4 planes)
+template <typename A, typename B>
+__global__ void convert_planes_kernel(const A* a, B* b, int M, int N, int P, int PS) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  const int o = y * P + x;
+  for (int k = 0; k < 4; ++k) b[k * PS + o] = (B)a[k * PS + o];
 }
 
 // --------------------------------------------------------------------------
@@ -344,49 +360,79 @@ hipError_t launch_metrics(const float* vx, const float* vy, int M, int N, int P,
   return hipGetLastError();
 }
 int wave_speed_blocks(int M, int N) { return (M * N + 255) / 256; }
-hipError_t launch_wave_speed(const float* q, const float* vx, const float* vy, const DevParams* par,
-                             int M, int N, int P, int PS, float* ws, float* part, hipStream_t s) {
-  hipLaunchKernelGGL(wave_speed_kernel, dim3(wave_speed_blocks(M, N)), dim3(256), 0, s, q, vx, vy,
+template <typename R>
+hipError_t launch_wave_speed(const R* q, const R* vx, const R* vy, const DevParamsT<R>* par, int M,
+                             int N, int P, int PS, R* ws, R* part, hipStream_t s) {
+  hipLaunchKernelGGL(wave_speed_kernel<R>, dim3(wave_speed_blocks(M, N)), dim3(256), 0, s, q, vx, vy,
                      par, M, N, P, PS, ws, part);
   return hipGetLastError();
 }
-hipError_t launch_reduce_max(const float* part, int n, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(256), 0, s, part, n, out);
+template <typename R>
+hipError_t launch_reduce_max(const R* part, int n, R* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_max_kernel<R>, dim3(1), dim3(256), 0, s, part, n, out);
   return hipGetLastError();
 }
 hipError_t launch_reduce_sum_f64(const double* part, int n, double* out, hipStream_t s) {
   hipLaunchKernelGGL(reduce_sum_f64_kernel, dim3(1), dim3(256), 0, s, part, n, out);
   return hipGetLastError();
 }
-hipError_t launch_forces(const float* q, const float* vx, const float* vy, const DevParams* par,
-                         int M, int PS, double ref_len, double* seg, double* out, double* cp,
+template <typename R>
+hipError_t launch_forces(const R* q, const R* vx, const R* vy, const DevParamsT<R>* par, int M,
+                         int PS, double ref_len, double* seg, double* out, double* cp,
                          hipStream_t s) {
-  hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(256), 0, s, q, vx, vy, par, M, PS, ref_len, seg,
-                     out, cp);
+  hipLaunchKernelGGL(forces_kernel<R>, dim3(1), dim3(256), 0, s, q, vx, vy, par, M, PS, ref_len,
+                     seg, out, cp);
   return hipGetLastError();
 }
-hipError_t launch_aos_to_soa(const float* aos, float* q, int M, int N, int P, int PS,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(aos_to_soa_kernel, rows_grid(M, N), dim3(256), 0, s, aos, q, M, N, P, PS);
+template <typename R>
+hipError_t launch_aos_to_soa(const R* aos, R* q, int M, int N, int P, int PS, hipStream_t s) {
+  hipLaunchKernelGGL(aos_to_soa_kernel<R>, rows_grid(M, N), dim3(256), 0, s, aos, q, M, N, P, PS);
   return hipGetLastError();
 }
-hipError_t launch_soa_to_aos(const float* q, const float* vx, const float* vy,
-                             const DevParams* par, int M, int N, int P, int PS, float* aos,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(soa_to_aos_kernel, rows_grid(M, N + 3), dim3(256), 0, s, q, vx, vy, par, M, N,
-                     P, PS, aos);
+template <typename R>
+hipError_t launch_soa_to_aos(const R* q, const R* vx, const R* vy, const DevParamsT<R>* par, int M,
+                             int N, int P, int PS, R* aos, hipStream_t s) {
+  hipLaunchKernelGGL(soa_to_aos_kernel<R>, rows_grid(M, N + 3), dim3(256), 0, s, q, vx, vy, par, M,
+                     N, P, PS, aos);
   return hipGetLastError();
 }
-hipError_t launch_planes_to_aos(const float* q, int M, int N, int P, int PS, float* aos,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(planes_to_aos_kernel, rows_grid(M, N), dim3(256), 0, s, q, M, N, P, PS, aos);
+template <typename R>
+hipError_t launch_planes_to_aos(const R* q, int M, int N, int P, int PS, R* aos, hipStream_t s) {
+  hipLaunchKernelGGL(planes_to_aos_kernel<R>, rows_grid(M, N), dim3(256), 0, s, q, M, N, P, PS, aos);
   return hipGetLastError();
 }
-hipError_t launch_fill_state(float* q, const DevParams* par, int M, int N, int P, int PS,
+template <typename R>
+hipError_t launch_fill_state(R* q, const DevParamsT<R>* par, int M, int N, int P, int PS,
                              int freestream, hipStream_t s) {
-  hipLaunchKernelGGL(fill_state_kernel, rows_grid(M, N), dim3(256), 0, s, q, par, M, N, P, PS,
+  hipLaunchKernelGGL(fill_state_kernel<R>, rows_grid(M, N), dim3(256), 0, s, q, par, M, N, P, PS,
                      freestream);
   return hipGetLastError();
 }
+template <typename A, typename B>
+hipError_t launch_convert_planes(const A* a, B* b, int M, int N, int P, int PS, hipStream_t s) {
+  hipLaunchKernelGGL((convert_planes_kernel<A, B>), rows_grid(M, N), dim3(256), 0, s, a, b, M, N, P,
+                     PS);
+  return hipGetLastError();
+}
+
+#define AUSM_AUX_INSTANTIATE(R)                                                                   \
+  template hipError_t launch_wave_speed<R>(const R*, const R*, const R*, const DevParamsT<R>*, int, \
+                                           int, int, int, R*, R*, hipStream_t);                 \
+  template hipError_t launch_reduce_max<R>(const R*, int, R*, hipStream_t);                      \
+  template hipError_t launch_forces<R>(const R*, const R*, const R*, const DevParamsT<R>*, int,    \
+                                       int, double, double*, double*, double*, hipStream_t);     \
+  template hipError_t launch_aos_to_soa<R>(const R*, R*, int, int, int, int, hipStream_t);        \
+  template hipError_t launch_soa_to_aos<R>(const R*, const R*, const R*, const DevParamsT<R>*, int, \
+                                           int, int, int, R*, hipStream_t);                     \
+  template hipError_t launch_planes_to_aos<R>(const R*, int, int, int, int, R*, hipStream_t);     \
+  template hipError_t launch_fill_state<R>(R*, const DevParamsT<R>*, int, int, int, int, int,     \
+                                           hipStream_t);
+AUSM_AUX_INSTANTIATE(float)
+AUSM_AUX_INSTANTIATE(double)
+#undef AUSM_AUX_INSTANTIATE
+template hipError_t launch_convert_planes<float, double>(const float*, double*, int, int, int, int,
+                                                         hipStream_t);
+template hipError_t launch_convert_planes<double, float>(const double*, float*, int, int, int, int,
+                                                         hipStream_t);
 
 }  // namespace ausm

@@ -238,7 +238,7 @@ static int seed_ws(ausm_batch* b, Slot* s) {
       return set_error(AUSM_ERR_NOMEM, "scratch allocation failed");
   }
   const Layout L = layout_of(M, N);
-  if (launch_wave_speed(s->dev.q[0], s->dev.vx, s->dev.vy, s->dpar, M, N, L.P, L.PS, nullptr,
+  if (launch_wave_speed(s->dev.q[0], s->dev.vx, s->dev.vy, s->dpar, M, N, L.P, L.PS, (float*)nullptr,
                         b->d_tmp, b->stream) != hipSuccess ||
       launch_batch_seed_ws(b->d_tmp, nb, s->dev.ws_part, L.nws_max, b->stream) != hipSuccess)
     return set_error(AUSM_ERR_HIP, "wave-speed seeding failed");

@@ -45,10 +45,18 @@ struct ausm_ctx {
   long long steps = 0;
   int since_refresh = 0;  // steps since the wave speed was last refreshed
   std::vector<ausm::GraphEntry> graphs;
+  // all-FP64 solve (allocated on the first switch to FP64)
+  int prec = 0;                                   // AUSM_PREC_FP32 / AUSM_PREC_FP64
+  ausm::DevParamsT<double> hpar64{};              // free stream unrounded
+  ausm::DevParamsT<double>* dpar64 = nullptr;
+  double* q64[3] = {nullptr, nullptr, nullptr};   // Q^n, Q1, Q2 in FP64
+  double* ws_part64 = nullptr;
+  double* stage_buf64 = nullptr;                  // AoS staging, (N+3) x M x 4 doubles
 };
 
 namespace ausm {
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 StageArgs stage_args(ausm_ctx* c, int ty, int stage);
+StageArgs64 stage_args64(ausm_ctx* c, int ty, int stage);
 }  // namespace ausm

@@ -26,41 +26,54 @@ enum { LIM_VANALBADA = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_NONE = 3 };
 // Solver parameters resident in device memory so that captured graphs pick
 // up changes without being rebuilt (the reference's Uniforms buffer,
 // src/util.js:2-27).
-struct DevParams {
-  float gamma, inRho, inP, vx, vy;
-  float inState[4];
-  float maxdt, cfl, Kp, Ku;
+template <typename R>
+struct DevParamsT {
+  R gamma, inRho, inP, vx, vy;
+  R inState[4];
+  R maxdt, cfl, Kp, Ku;
   // derived in double on the host, used by the fast path only
-  float gm1;       // gamma - 1
-  float ggm1;      // gamma / (gamma - 1)
-  float minf_coef; // |V|^2 / (gamma * p_inf)
-  float astar_coef; // 2 (gamma - 1) / (gamma + 1)
+  R gm1;        // gamma - 1
+  R ggm1;       // gamma / (gamma - 1)
+  R minf_coef;  // |V|^2 / (gamma * p_inf)
+  R astar_coef; // 2 (gamma - 1) / (gamma + 1)
 };
+using DevParams = DevParamsT<float>;
 
-struct Prim {
-  float r, u, v, p;
+template <typename R>
+struct PrimT {
+  R r, u, v, p;
 };
-struct Cons {
-  float r, mu, mv, e;
+template <typename R>
+struct ConsT {
+  R r, mu, mv, e;
 };
-struct Flux {
-  float f0, f1, f2, f3;
+template <typename R>
+struct FluxT {
+  R f0, f1, f2, f3;
 };
+using Prim = PrimT<float>;
+using Cons = ConsT<float>;
+using Flux = FluxT<float>;
 
-__device__ __forceinline__ float wstep(float edge, float x) { return x >= edge ? 1.0f : 0.0f; }
-__device__ __forceinline__ float wsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
-__device__ __forceinline__ float wsat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
-__device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+template <typename R>
+__device__ __forceinline__ R wstep(R edge, R x) { return x >= edge ? R(1) : R(0); }
+template <typename R>
+__device__ __forceinline__ R wsign(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : R(0)); }
+template <typename R>
+__device__ __forceinline__ R wsat(R x) { return fmin(fmax(x, R(0)), R(1)); }
+template <typename R>
+__device__ __forceinline__ R wmix(R a, R b, R t) { return a * (R(1) - t) + b * t; }
 
 // toPrimitive (ausm.js:145-150)
-__device__ __forceinline__ Prim to_prim_exact(Cons q, float gamma) {
-  Prim w;
+template <typename R>
+__device__ __forceinline__ PrimT<R> to_prim_exact(ConsT<R> q, R gamma) {
+  PrimT<R> w;
   w.r = q.r;
   w.u = q.mu / q.r;
   w.v = q.mv / q.r;
-  const float d = w.u * w.u + w.v * w.v;
-  const float p = (q.e - (0.5f * q.r) * d) * (gamma - 1.0f);
-  w.p = fmaxf(1e-7f, p);
+  const R d = w.u * w.u + w.v * w.v;
+  const R p = (q.e - (R(0.5) * q.r) * d) * (gamma - R(1.0));
+  w.p = fmax(R(1e-7), p);
   return w;
 }
 
@@ -78,9 +91,10 @@ __device__ __forceinline__ Prim to_prim_fast(Cons q, float gm1) {
 
 // face unit normal: normalize(d) rotated by +90 degrees; also returns |d|
 // (the face length the residual uses, shaders/grid.js:143-146)
-__device__ __forceinline__ void normal_exact(float dx, float dy, float& nx, float& ny, float& len) {
-  len = sqrtf(dx * dx + dy * dy);
-  const float tx = dx / len, ty = dy / len;
+template <typename R>
+__device__ __forceinline__ void normal_exact(R dx, R dy, R& nx, R& ny, R& len) {
+  len = sqrt(dx * dx + dy * dy);
+  const R tx = dx / len, ty = dy / len;
   nx = -ty;
   ny = tx;
 }
@@ -95,139 +109,144 @@ __device__ __forceinline__ void normal_fast(float dx, float dy, float& nx, float
 // ---------------------------------------------------------------------------
 // Exact face flux (ausm.js:183-393)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float van_albada_exact(float r) {
-  const float r2 = r * r;
-  return (r2 + r) / (r2 + 1.0f);
+template <typename R>
+__device__ __forceinline__ R van_albada_exact(R r) {
+  const R r2 = r * r;
+  return (r2 + r) / (r2 + R(1.0));
 }
-__device__ __forceinline__ float van_leer_exact(float r) {
-  const float a = fabsf(r);
-  return (r + a) / (1.0f + a);
+template <typename R>
+__device__ __forceinline__ R van_leer_exact(R r) {
+  const R a = fabs(r);
+  return (r + a) / (R(1.0) + a);
 }
-__device__ __forceinline__ float minmod_exact(float a, float b) {
-  return (0.5f * (wsign(a) + wsign(b))) * fminf(fabsf(a), fabsf(b));
+template <typename R>
+__device__ __forceinline__ R minmod_exact(R a, R b) {
+  return (R(0.5) * (wsign(a) + wsign(b))) * fmin(fabs(a), fabs(b));
 }
 
 // MUSCL on one primitive component (ausm.js:240-249)
-__device__ __forceinline__ void muscl_exact(int limiter, float wl2, float wl, float wr, float wr2,
-                                            float& L, float& Rr) {
-  const float eps = 1e-6f;
-  const float dL = wl - wl2;
-  const float dF = wr - wl;
-  const float dR = wr2 - wr;
+template <typename R>
+__device__ __forceinline__ void muscl_exact(int limiter, R wl2, R wl, R wr, R wr2,
+                                            R& L, R& Rr) {
+  const R eps = R(1e-6);
+  const R dL = wl - wl2;
+  const R dF = wr - wl;
+  const R dR = wr2 - wr;
   if (limiter == LIM_MINMOD) {
-    L = wl + 0.5f * minmod_exact(dL, dF);
-    Rr = wr - 0.5f * minmod_exact(dF, dR);
+    L = wl + R(0.5) * minmod_exact(dL, dF);
+    Rr = wr - R(0.5) * minmod_exact(dF, dR);
   } else {
-    const float rL = (dL * dF + eps) / (dF * dF + eps);
-    const float rR = (dF * dR + eps) / (dF * dF + eps);
-    const float phiL = limiter == LIM_VANLEER ? van_leer_exact(rL) : van_albada_exact(rL);
-    const float phiR = limiter == LIM_VANLEER ? van_leer_exact(rR) : van_albada_exact(rR);
-    L = wl + (0.5f * phiL) * dF;
-    Rr = wr - (0.5f * phiR) * dF;
+    const R rL = (dL * dF + eps) / (dF * dF + eps);
+    const R rR = (dF * dR + eps) / (dF * dF + eps);
+    const R phiL = limiter == LIM_VANLEER ? van_leer_exact(rL) : van_albada_exact(rL);
+    const R phiR = limiter == LIM_VANLEER ? van_leer_exact(rR) : van_albada_exact(rR);
+    L = wl + (R(0.5) * phiL) * dF;
+    Rr = wr - (R(0.5) * phiR) * dF;
   }
 }
 
-template <int SCHEME>
-__device__ __forceinline__ Flux face_flux_exact(const DevParams& P, int limiter, Prim a, Prim b,
-                                                Prim c, Prim d, float nx, float ny) {
+template <int SCHEME, typename Real>
+__device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P, int limiter, PrimT<Real> a,
+                                                   PrimT<Real> b, PrimT<Real> c, PrimT<Real> d, Real nx,
+                                                   Real ny) {
   // a = W_L2, b = W_L, c = W_R, d = W_R2
-  const float gamma = P.gamma;
-  const float gammaMinus1 = gamma - 1.0f;
-  Prim L = b, R = c;
+  const Real gamma = P.gamma;
+  const Real gammaMinus1 = gamma - Real(1.0);
+  PrimT<Real> L = b, R = c;
   if (limiter != LIM_NONE) {
     muscl_exact(limiter, a.r, b.r, c.r, d.r, L.r, R.r);
     muscl_exact(limiter, a.u, b.u, c.u, d.u, L.u, R.u);
     muscl_exact(limiter, a.v, b.v, c.v, d.v, L.v, R.v);
     muscl_exact(limiter, a.p, b.p, c.p, d.p, L.p, R.p);
   }
-  const float u_L = L.u * nx + L.v * ny;
-  const float u_R = R.u * nx + R.v * ny;
-  const float vvL = L.u * L.u + L.v * L.v;
-  const float vvR = R.u * R.u + R.v * R.v;
-  const float h_L = (L.p / (gammaMinus1 * L.r) + 0.5f * vvL) + L.p / L.r;
-  const float h_R = (R.p / (gammaMinus1 * R.r) + 0.5f * vvR) + R.p / R.r;
-  float mdot, P_i;
+  const Real u_L = L.u * nx + L.v * ny;
+  const Real u_R = R.u * nx + R.v * ny;
+  const Real vvL = L.u * L.u + L.v * L.v;
+  const Real vvR = R.u * R.u + R.v * R.v;
+  const Real h_L = (L.p / (gammaMinus1 * L.r) + Real(0.5) * vvL) + L.p / L.r;
+  const Real h_R = (R.p / (gammaMinus1 * R.r) + Real(0.5) * vvR) + R.p / R.r;
+  Real mdot, P_i;
   if constexpr (SCHEME == SCHEME_AUSMUP) {
     // ausm.js:273-327
-    const float M_inf2 = (P.vx * P.vx + P.vy * P.vy) / ((gamma * P.inP) / L.r);
-    const float sigma = 1.0f;
-    const float h_t = 0.5f * (h_L + h_R);
-    const float aStar = sqrtf(fmaxf(1e-6f, ((2.0f * gammaMinus1) / (gamma + 1.0f)) * h_t));
-    const float aHat_L = (aStar * aStar) / fmaxf(u_L, aStar);
-    const float aHat_R = (aStar * aStar) / fmaxf(-u_R, aStar);
-    const float a_i = fminf(aHat_L, aHat_R);
-    const float M_L = u_L / a_i;
-    const float M_R = u_R / a_i;
-    const float a_i2 = a_i * a_i;
-    const float Mbar2 = (u_L * u_L + u_R * u_R) / (a_i2 * 2.0f);
-    const float M_o2 = fminf(1.0f, fmaxf(Mbar2, M_inf2));
-    const float f_a = M_o2 * (2.0f - M_o2);
-    const float rho_i = 0.5f * (L.r + R.r);
-    const float Mp1L = M_L + 1.0f, Mp1R = M_R + 1.0f;
-    const float Mm1L = M_L - 1.0f, Mm1R = M_R - 1.0f;
-    const float M2pL = (0.25f * Mp1L) * Mp1L, M2pR = (0.25f * Mp1R) * Mp1R;
-    const float M2mL = (-0.25f * Mm1L) * Mm1L, M2mR = (-0.25f * Mm1R) * Mm1R;
-    const float M4subL = M2pL * (1.0f + 2.0f * M2mL);
-    const float M4subR = M2mR * (1.0f - 2.0f * M2pR);
-    const float supL = wstep(1.0f, fabsf(M_L)), supR = wstep(1.0f, fabsf(M_R));
-    const float M4L = wmix(M4subL, fmaxf(0.0f, M_L), supL);
-    const float M4R = wmix(M4subR, fminf(0.0f, M_R), supR);
-    float kp = P.Kp / f_a;
-    kp = kp * fmaxf(1.0f - sigma * Mbar2, 0.0f);
+    const Real M_inf2 = (P.vx * P.vx + P.vy * P.vy) / ((gamma * P.inP) / L.r);
+    const Real sigma = Real(1.0);
+    const Real h_t = Real(0.5) * (h_L + h_R);
+    const Real aStar = sqrt(fmax(Real(1e-6), ((Real(2.0) * gammaMinus1) / (gamma + Real(1.0))) * h_t));
+    const Real aHat_L = (aStar * aStar) / fmax(u_L, aStar);
+    const Real aHat_R = (aStar * aStar) / fmax(-u_R, aStar);
+    const Real a_i = fmin(aHat_L, aHat_R);
+    const Real M_L = u_L / a_i;
+    const Real M_R = u_R / a_i;
+    const Real a_i2 = a_i * a_i;
+    const Real Mbar2 = (u_L * u_L + u_R * u_R) / (a_i2 * Real(2.0));
+    const Real M_o2 = fmin(Real(1.0), fmax(Mbar2, M_inf2));
+    const Real f_a = M_o2 * (Real(2.0) - M_o2);
+    const Real rho_i = Real(0.5) * (L.r + R.r);
+    const Real Mp1L = M_L + Real(1.0), Mp1R = M_R + Real(1.0);
+    const Real Mm1L = M_L - Real(1.0), Mm1R = M_R - Real(1.0);
+    const Real M2pL = (Real(0.25) * Mp1L) * Mp1L, M2pR = (Real(0.25) * Mp1R) * Mp1R;
+    const Real M2mL = (-Real(0.25) * Mm1L) * Mm1L, M2mR = (-Real(0.25) * Mm1R) * Mm1R;
+    const Real M4subL = M2pL * (Real(1.0) + Real(2.0) * M2mL);
+    const Real M4subR = M2mR * (Real(1.0) - Real(2.0) * M2pR);
+    const Real supL = wstep(Real(1.0), fabs(M_L)), supR = wstep(Real(1.0), fabs(M_R));
+    const Real M4L = wmix(M4subL, fmax(Real(0.0), M_L), supL);
+    const Real M4R = wmix(M4subR, fmin(Real(0.0), M_R), supR);
+    Real kp = P.Kp / f_a;
+    kp = kp * fmax(Real(1.0) - sigma * Mbar2, Real(0.0));
     kp = kp * (R.p - L.p);
     kp = kp / (rho_i * a_i2);
-    const float M_i = (M4L + M4R) - kp;
-    const float alpha4 = 0.75f * (-4.0f + (5.0f * f_a) * f_a);
-    const float P5subL = M2pL * ((2.0f - M_L) + (alpha4 * M_L) * M2mL);
-    const float P5subR = M2mR * ((-2.0f - M_R) - (alpha4 * M_R) * M2pR);
-    const float P5L = wmix(P5subL, wstep(0.0f, M_L), supL);
-    const float P5R = wmix(P5subR, wstep(0.0f, -M_R), supR);
-    float ku = P.Ku * P5L;
+    const Real M_i = (M4L + M4R) - kp;
+    const Real alpha4 = Real(0.75) * (-Real(4.0) + (Real(5.0) * f_a) * f_a);
+    const Real P5subL = M2pL * ((Real(2.0) - M_L) + (alpha4 * M_L) * M2mL);
+    const Real P5subR = M2mR * ((-Real(2.0) - M_R) - (alpha4 * M_R) * M2pR);
+    const Real P5L = wmix(P5subL, wstep(Real(0.0), M_L), supL);
+    const Real P5R = wmix(P5subR, wstep(Real(0.0), -M_R), supR);
+    Real ku = P.Ku * P5L;
     ku = ku * P5R;
     ku = ku * (L.r + R.r);
     ku = ku * f_a;
     ku = ku * a_i;
     ku = ku * (u_R - u_L);
     P_i = (P5L * L.p + P5R * R.p) - ku;
-    mdot = (a_i * M_i) * (M_i > 0.0f ? L.r : R.r);
+    mdot = (a_i * M_i) * (M_i > Real(0.0) ? L.r : R.r);
   } else {
     // ausm.js:348-384
-    const float a_L = sqrtf((gamma * fmaxf(L.p, 1e-10f)) / fmaxf(L.r, 1e-10f));
-    const float a_R = sqrtf((gamma * fmaxf(R.p, 1e-10f)) / fmaxf(R.r, 1e-10f));
-    const float a_i = (a_L + a_R) * 0.5f;
-    const float M_L = u_L / a_i;
-    const float M_R = u_R / a_i;
-    const float Uarc = sqrtf(0.5f * (vvL + vvR));
-    const float Marc = fminf(1.0f, Uarc / a_i);
-    const float X = (1.0f - Marc) * (1.0f - Marc);
-    const float qL = M_L + 1.0f, qR = M_R - 1.0f;
-    const float sqL = (0.5f * qL) * qL, sqR = (0.5f * qR) * qR;
-    const float supL = wstep(1.0f, fabsf(M_L)), supR = wstep(1.0f, fabsf(M_R));
-    const float fpp = wsat(0.5f * wmix(sqL * (2.0f - 1.0f * M_L), 1.0f + 1.0f * wsign(M_L), supL));
-    const float fpm =
-        wsat(0.5f * wmix(sqR * (2.0f - -1.0f * M_R), 1.0f + -1.0f * wsign(M_R), supR));
+    const Real a_L = sqrt((gamma * fmax(L.p, Real(1e-10))) / fmax(L.r, Real(1e-10)));
+    const Real a_R = sqrt((gamma * fmax(R.p, Real(1e-10))) / fmax(R.r, Real(1e-10)));
+    const Real a_i = (a_L + a_R) * Real(0.5);
+    const Real M_L = u_L / a_i;
+    const Real M_R = u_R / a_i;
+    const Real Uarc = sqrt(Real(0.5) * (vvL + vvR));
+    const Real Marc = fmin(Real(1.0), Uarc / a_i);
+    const Real X = (Real(1.0) - Marc) * (Real(1.0) - Marc);
+    const Real qL = M_L + Real(1.0), qR = M_R - Real(1.0);
+    const Real sqL = (Real(0.5) * qL) * qL, sqR = (Real(0.5) * qR) * qR;
+    const Real supL = wstep(Real(1.0), fabs(M_L)), supR = wstep(Real(1.0), fabs(M_R));
+    const Real fpp = wsat(Real(0.5) * wmix(sqL * (Real(2.0) - Real(1.0) * M_L), Real(1.0) + Real(1.0) * wsign(M_L), supL));
+    const Real fpm =
+        wsat(Real(0.5) * wmix(sqR * (Real(2.0) - -Real(1.0) * M_R), Real(1.0) + -Real(1.0) * wsign(M_R), supR));
     if constexpr (SCHEME == SCHEME_SLAU2) {
-      const float t1 = (fpp - fpm) * (L.p - R.p);
-      const float t2 = ((Uarc * ((fpp + fpm) - 1.0f)) * (L.r + R.r)) * a_i;
-      P_i = 0.5f * ((t1 + (L.p + R.p)) + t2);
+      const Real t1 = (fpp - fpm) * (L.p - R.p);
+      const Real t2 = ((Uarc * ((fpp + fpm) - Real(1.0))) * (L.r + R.r)) * a_i;
+      P_i = Real(0.5) * ((t1 + (L.p + R.p)) + t2);
     } else {
-      const float t1 = (fpp - fpm) * (L.p - R.p);
-      const float t2 = (1.0f + (1.0f - X) * ((fpp + fpm) - 1.0f)) * (L.p + R.p);
-      P_i = 0.5f * (t1 + t2);
+      const Real t1 = (fpp - fpm) * (L.p - R.p);
+      const Real t2 = (Real(1.0) + (Real(1.0) - X) * ((fpp + fpm) - Real(1.0))) * (L.p + R.p);
+      P_i = Real(0.5) * (t1 + t2);
     }
-    const float delta_p = R.p - L.p;
-    const float ubar_n = (L.r * fabsf(u_L) + R.r * fabsf(u_R)) / (L.r + R.r);
-    const float g = wsat(-M_L) * wsat(M_R);
-    const float ubp = wmix(ubar_n, fabsf(u_L), g);
-    const float ubm = wmix(ubar_n, fabsf(u_R), g);
-    mdot = 0.5f * ((L.r * (u_L + ubp) + R.r * (u_R - ubm)) * (1.0f - g) - (X / a_i) * delta_p);
+    const Real delta_p = R.p - L.p;
+    const Real ubar_n = (L.r * fabs(u_L) + R.r * fabs(u_R)) / (L.r + R.r);
+    const Real g = wsat(-M_L) * wsat(M_R);
+    const Real ubp = wmix(ubar_n, fabs(u_L), g);
+    const Real ubm = wmix(ubar_n, fabs(u_R), g);
+    mdot = Real(0.5) * ((L.r * (u_L + ubp) + R.r * (u_R - ubm)) * (Real(1.0) - g) - (X / a_i) * delta_p);
   }
-  const bool left = mdot > 0.0f;
-  Flux F;
-  F.f0 = 1.0f * mdot + 0.0f;
+  const bool left = mdot > Real(0.0);
+  FluxT<Real> F;
+  F.f0 = Real(1.0) * mdot + Real(0.0);
   F.f1 = (left ? L.u : R.u) * mdot + nx * P_i;
   F.f2 = (left ? L.v : R.v) * mdot + ny * P_i;
-  F.f3 = (left ? h_L : h_R) * mdot + 0.0f;
+  F.f3 = (left ? h_L : h_R) * mdot + Real(0.0);
   return F;
 }
 
@@ -371,12 +390,13 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
 
 // slip wall: reflect the velocity of interior row 1 about the body-segment
 // normal (ausm.js:57-64).  (nx, ny) = rot90(normalize(v_i - v_{i+1})).
-__device__ __forceinline__ Cons wall_ghost(Cons I, float nx, float ny) {
-  const float rho = I.r;
-  const float ux = I.mu / rho, uy = I.mv / rho;
-  const float k = 2.0f * (nx * ux + ny * uy);
-  const float gx = ux - k * nx, gy = uy - k * ny;
-  Cons g;
+template <typename R>
+__device__ __forceinline__ ConsT<R> wall_ghost(ConsT<R> I, R nx, R ny) {
+  const R rho = I.r;
+  const R ux = I.mu / rho, uy = I.mv / rho;
+  const R k = R(2.0) * (nx * ux + ny * uy);
+  const R gx = ux - k * nx, gy = uy - k * ny;
+  ConsT<R> g;
   g.r = rho;
   g.mu = gx * rho;
   g.mv = gy * rho;
@@ -386,60 +406,61 @@ __device__ __forceinline__ Cons wall_ghost(Cons I, float nx, float ny) {
 
 // far field from interior row N (ausm.js:65-117).  second = the outer ghost
 // row (N+2), which extrapolates linearly on outflow.
-__device__ __forceinline__ Cons far_ghost(const DevParams& P, Cons I, float nx, float ny,
+template <typename R>
+__device__ __forceinline__ ConsT<R> far_ghost(const DevParamsT<R>& P, ConsT<R> I, R nx, R ny,
                                           bool second) {
-  const float gamma = P.gamma;
-  const float rho = I.r;
-  const float ux = I.mu / rho, uy = I.mv / rho;
-  const float bio = nx * P.vx + ny * P.vy;
-  Cons g;
-  if (bio < 0.0f) {
+  const R gamma = P.gamma;
+  const R rho = I.r;
+  const R ux = I.mu / rho, uy = I.mv / rho;
+  const R bio = nx * P.vx + ny * P.vy;
+  ConsT<R> g;
+  if (bio < R(0.0)) {
     g.r = P.inState[0];
     g.mu = P.inState[1];
     g.mv = P.inState[2];
     g.e = P.inState[3];
     return g;
   }
-  const float gm1inv = 1.0f / (gamma - 1.0f);
-  const float u2 = ux * ux + uy * uy;
-  const float Pr = (I.e - (0.5f * u2) * rho) * (gamma - 1.0f);
-  const float a2 = (gamma * Pr) / rho;
-  const float a2inf = (gamma * P.inP) / P.inRho;
-  const float M_int = u2 / a2;
-  const float M_inf2 = (P.vx * P.vx + P.vy * P.vy) / a2inf;
-  if (M_int < 1.0f && M_inf2 < 1.0f) {
-    const float un_int = ux * nx + uy * ny;
-    const float un_inf = bio;
-    const float a_int = sqrtf(a2);
-    const float a_inf = sqrtf(a2inf);
-    const float J_int = un_int + (2.0f * a_int) * gm1inv;
-    const float J_ext = un_inf - (2.0f * a_inf) * gm1inv;
-    const float un_g = 0.5f * (J_int + J_ext);
-    const float a_g = fmaxf(a_int * 0.1f, (0.25f * (J_int - J_ext)) * (gamma - 1.0f));
-    const float X = fmaxf(1e-6f, a_g / a_int);
-    const float X2 = X * X;
-    const float rho_g = fmaxf(1e-6f, ((rho * X2) * X2) * X);
-    const float dn = un_g - un_int;
-    const float vgx = dn * nx + ux, vgy = dn * ny + uy;
-    const float rhoE_g = fmaxf(1e-6f, ((rho_g * a_g) * a_g) / gamma) * gm1inv +
-                         (0.5f * rho_g) * (vgx * vgx + vgy * vgy);
+  const R gm1inv = R(1.0) / (gamma - R(1.0));
+  const R u2 = ux * ux + uy * uy;
+  const R Pr = (I.e - (R(0.5) * u2) * rho) * (gamma - R(1.0));
+  const R a2 = (gamma * Pr) / rho;
+  const R a2inf = (gamma * P.inP) / P.inRho;
+  const R M_int = u2 / a2;
+  const R M_inf2 = (P.vx * P.vx + P.vy * P.vy) / a2inf;
+  if (M_int < R(1.0) && M_inf2 < R(1.0)) {
+    const R un_int = ux * nx + uy * ny;
+    const R un_inf = bio;
+    const R a_int = sqrt(a2);
+    const R a_inf = sqrt(a2inf);
+    const R J_int = un_int + (R(2.0) * a_int) * gm1inv;
+    const R J_ext = un_inf - (R(2.0) * a_inf) * gm1inv;
+    const R un_g = R(0.5) * (J_int + J_ext);
+    const R a_g = fmax(a_int * R(0.1), (R(0.25) * (J_int - J_ext)) * (gamma - R(1.0)));
+    const R X = fmax(R(1e-6), a_g / a_int);
+    const R X2 = X * X;
+    const R rho_g = fmax(R(1e-6), ((rho * X2) * X2) * X);
+    const R dn = un_g - un_int;
+    const R vgx = dn * nx + ux, vgy = dn * ny + uy;
+    const R rhoE_g = fmax(R(1e-6), ((rho_g * a_g) * a_g) / gamma) * gm1inv +
+                         (R(0.5) * rho_g) * (vgx * vgx + vgy * vgy);
     g.r = rho_g;
     g.mu = vgx * rho_g;
     g.mv = vgy * rho_g;
     g.e = rhoE_g;
-  } else if (M_inf2 > 1.0f) {
+  } else if (M_inf2 > R(1.0)) {
     g.r = I.r;
     g.mu = I.mu;
     g.mv = I.mv;
-    g.e = P.inP / (gamma - 1.0f) + (0.5f * rho) * u2;
+    g.e = P.inP / (gamma - R(1.0)) + (R(0.5) * rho) * u2;
   } else {
     g = I;
   }
   if (second) {
-    g.r = 2.0f * g.r - I.r;
-    g.mu = 2.0f * g.mu - I.mu;
-    g.mv = 2.0f * g.mv - I.mv;
-    g.e = 2.0f * g.e - I.e;
+    g.r = R(2.0) * g.r - I.r;
+    g.mu = R(2.0) * g.mu - I.mu;
+    g.mv = R(2.0) * g.mv - I.mv;
+    g.e = R(2.0) * g.e - I.e;
   }
   return g;
 }

@@ -12,18 +12,21 @@ constexpr int kWavesPerBlock = 4;  // wavefronts per stage-kernel workgroup
 
 // One launch of the fused RK-stage kernel.  State buffers are SoA planes
 // (rho, rho*u, rho*v, rho*E) of N interior rows with row pitch P and plane
-// stride PS; vertices are two planes (x, y) of N+1 rows with pitch P.
-struct StageArgs {
-  const float* qin;   // stage input Q^n, Q1 or Q2 (read with a 2-cell halo)
-  const float* qn;    // Q^n (read at the cell only; stages 2 and 3)
-  float* qout;        // Q1, Q2 or Q^{n+1} (may be qn: written after its own read)
-  const float* vx;
-  const float* vy;
-  const DevParams* par;
-  const float* ws_in;  // per-block max wave speeds of the last refresh
-  float* ws_out;       // per-block max wave speeds of this stage's output (stage 3)
-  double* res_out;     // sums of squares of the density residual per (row, strip), N x tiles_x (stage 1)
-  float* maxws_cur;    // max wave speed the current step's dt is based on
+// stride PS; vertices are two planes (x, y) of N+1 rows with pitch P.  R is
+// the arithmetic type of the state, the vertices and the stage arithmetic
+// (float; double for the all-FP64 solve).
+template <typename R>
+struct StageArgsT {
+  const R* qin;   // stage input Q^n, Q1 or Q2 (read with a 2-cell halo)
+  const R* qn;    // Q^n (read at the cell only; stages 2 and 3)
+  R* qout;        // Q1, Q2 or Q^{n+1} (may be qn: written after its own read)
+  const R* vx;
+  const R* vy;
+  const DevParamsT<R>* par;
+  const R* ws_in;    // per-block max wave speeds of the last refresh
+  R* ws_out;         // per-block max wave speeds of this stage's output (stage 3)
+  double* res_out;   // sums of squares of the density residual per (row, strip), N x tiles_x (stage 1)
+  R* maxws_cur;      // max wave speed the current step's dt is based on
   int n_ws_in;
   int M, N, P, PS;
   int stage;    // 0 = residual only (written to qout), 1, 2, 3
@@ -32,9 +35,13 @@ struct StageArgs {
   int refresh;  // stage 3: write ws_out
   int tiles_x;  // wavefront tiles around i
 };
+using StageArgs = StageArgsT<float>;
+using StageArgs64 = StageArgsT<double>;
 
 int stage_blocks(int M, int N, int ty);
 int stage_tiles_x(int M);
 hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s);
+// all-FP64 stage (exact arithmetic only); ty in {1, 2, 4}
+hipError_t launch_stage64(const StageArgs64& a, int scheme, int ty, hipStream_t s);
 
 }  // namespace ausm

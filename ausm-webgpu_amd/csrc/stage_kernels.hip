@@ -35,6 +35,19 @@ __global__ __launch_bounds__(256) void stage_kernel(const StageArgs a) {
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
+// All-FP64 variant (exact arithmetic): the same tile walk over double
+// state and vertex planes, for the FP64 comparison solve.
+template <int SCHEME, int TY>
+__global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
+  const DevParamsT<double> P = *a.par;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = tile % a.tiles_x;
+  const int tyb = tile / a.tiles_x;
+  const double dt = stage_dt(P, a, blockIdx.x == 0);
+  const double ws_acc = stage_tile<SCHEME, false, TY, double>(P, a, tx, tyb, dt);
+  if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
+}
+
 int stage_tiles_x(int M) { return (M + kOwn - 1) / kOwn; }
 
 int stage_blocks(int M, int N, int ty) {
@@ -68,6 +81,27 @@ hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStr
       case SCHEME_SLAU: return launch_ty<SCHEME_SLAU, false>(a, ty, s);
       case SCHEME_AUSMUP: return launch_ty<SCHEME_AUSMUP, false>(a, ty, s);
     }
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int SCHEME>
+static hipError_t launch_ty64(const StageArgs64& a, int ty, hipStream_t s) {
+  const int nb = stage_blocks(a.M, a.N, ty);
+  switch (ty) {
+    case 1: hipLaunchKernelGGL((stage_kernel64<SCHEME, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel64<SCHEME, 2>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel64<SCHEME, 4>), dim3(nb), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_stage64(const StageArgs64& a, int scheme, int ty, hipStream_t s) {
+  switch (scheme) {
+    case SCHEME_SLAU2: return launch_ty64<SCHEME_SLAU2>(a, ty, s);
+    case SCHEME_SLAU: return launch_ty64<SCHEME_SLAU>(a, ty, s);
+    case SCHEME_AUSMUP: return launch_ty64<SCHEME_AUSMUP>(a, ty, s);
   }
   return hipErrorInvalidValue;
 }

@@ -22,16 +22,31 @@ __device__ __forceinline__ float from_left(float v) {
 __device__ __forceinline__ float from_right(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
 }
-__device__ __forceinline__ Prim from_left(Prim w) {
-  return Prim{from_left(w.r), from_left(w.u), from_left(w.v), from_left(w.p)};
+// doubles move as two 32-bit halves
+__device__ __forceinline__ double from_left(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ Prim from_right(Prim w) {
-  return Prim{from_right(w.r), from_right(w.u), from_right(w.v), from_right(w.p)};
+__device__ __forceinline__ double from_right(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+template <typename R>
+__device__ __forceinline__ PrimT<R> from_left(PrimT<R> w) {
+  return PrimT<R>{from_left(w.r), from_left(w.u), from_left(w.v), from_left(w.p)};
+}
+template <typename R>
+__device__ __forceinline__ PrimT<R> from_right(PrimT<R> w) {
+  return PrimT<R>{from_right(w.r), from_right(w.u), from_right(w.v), from_right(w.p)};
 }
 
-__device__ __forceinline__ float nan_max(float m, float x) { return (x > m || x != x) ? x : m; }
+template <typename R>
+__device__ __forceinline__ R nan_max(R m, R x) { return (x > m || x != x) ? x : m; }
 
-__device__ __forceinline__ float wave_max(float v) {
+template <typename R>
+__device__ __forceinline__ R wave_max(R v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v = nan_max(v, __shfl_xor(v, o));
   return v;
@@ -42,15 +57,15 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int SCHEME, bool FAST>
-__device__ __forceinline__ Flux face_flux(const DevParams& P, int lim, Prim a, Prim b, Prim c,
-                                          Prim d, float nx, float ny) {
+template <int SCHEME, bool FAST, typename R>
+__device__ __forceinline__ FluxT<R> face_flux(const DevParamsT<R>& P, int lim, PrimT<R> a,
+                                              PrimT<R> b, PrimT<R> c, PrimT<R> d, R nx, R ny) {
   if constexpr (FAST) return face_flux_fast<SCHEME>(P, lim, a, b, c, d, nx, ny);
   else return face_flux_exact<SCHEME>(P, lim, a, b, c, d, nx, ny);
 }
 
-template <bool FAST>
-__device__ __forceinline__ void face_normal(float dx, float dy, float& nx, float& ny, float& len) {
+template <bool FAST, typename R>
+__device__ __forceinline__ void face_normal(R dx, R dy, R& nx, R& ny, R& len) {
   if constexpr (FAST) normal_fast(dx, dy, nx, ny, len);
   else normal_exact(dx, dy, nx, ny, len);
 }
@@ -58,10 +73,15 @@ __device__ __forceinline__ void face_normal(float dx, float dy, float& nx, float
 // One wavefront's share of a workgroup tile: the strip of columns
 // [60*tx, 60*tx+60) and rows [(4*tyb + wave)*TY, ... + TY) of one grid.
 // Returns the wavefront's max wave speed of the new state (stage 3 with
-// refresh; 0 otherwise).  All 64 lanes must call it together.
-template <int SCHEME, bool FAST, int TY>
-__device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs& a, int tx, int tyb,
-                                            float dt) {
+// refresh; 0 otherwise).  All 64 lanes must call it together.  R = double
+// is the all-FP64 variant of the exact arithmetic.
+template <int SCHEME, bool FAST, int TY, typename R = float>
+__device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
+                                        int tyb, R dt) {
+  static_assert(!FAST || sizeof(R) == 4, "the fast arithmetic is FP32 only");
+  using Prim = PrimT<R>;
+  using Cons = ConsT<R>;
+  using Flux = FluxT<R>;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int M = a.M, N = a.N, Pp = a.P, PS = a.PS;
@@ -75,10 +95,10 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
   const int lim = a.limiter;
   const int stage = a.stage;
 
-  float ws_acc = 0.0f;
+  R ws_acc = R(0);
 
   if (j0 < N) {
-    const float* __restrict__ qin = a.qin;
+    const R* __restrict__ qin = a.qin;
     // ---- prefetch: every row the tile reads, issued before any arithmetic.
     // Stage-input rows j0-2 .. j0+TY+1 (clamped to the interior; the clamped
     // copies of rows 0 and N-1 are what the ghost rows are formed from),
@@ -91,7 +111,7 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
       const int o = j * Pp + col;
       raw[k] = Cons{qin[o], qin[PS + o], qin[2 * PS + o], qin[3 * PS + o]};
     }
-    float vxr[TY + 1], vyr[TY + 1];
+    R vxr[TY + 1], vyr[TY + 1];
 #pragma unroll
     for (int k = 0; k <= TY; ++k) {
       const int o = min(j0 + k, N) * Pp + col;
@@ -119,13 +139,13 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
     };
     Prim gW{}, gF1{}, gF2{};
     if (j0 < 2) {
-      float nx, ny, len;
+      R nx, ny, len;
       normal_exact(a.vx[col] - a.vx[colr], a.vy[col] - a.vy[colr], nx, ny, len);
       gW = to_prim(wall_ghost(row_of(0), nx, ny));
     }
     if (j0 + TY + 2 > N) {
       const int o = N * Pp;
-      float nx, ny, len;
+      R nx, ny, len;
       normal_exact(a.vx[o + col] - a.vx[o + colr], a.vy[o + col] - a.vy[o + colr], nx, ny, len);
       const Cons I = row_of(N - 1);
       gF1 = to_prim(far_ghost(P, I, nx, ny, false));
@@ -144,14 +164,14 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
     Prim w_m1 = prim_of(1);
     Prim w_0 = prim_of(2);
     Prim w_p1 = prim_of(3);
-    float vx0 = vxr[0], vy0 = vyr[0];
-    float vx0r = from_right(vx0), vy0r = from_right(vy0);
+    R vx0 = vxr[0], vy0 = vyr[0];
+    R vx0r = from_right(vx0), vy0r = from_right(vy0);
 
     // bottom face of row j0: vertical flux at face index j0 (ausm.js:191-205)
     Flux Gd;
-    float lD;
+    R lD;
     {
-      float nx, ny;
+      R nx, ny;
       face_normal<FAST>(vx0 - vx0r, vy0 - vy0r, nx, ny, lD);
       const Flux G = face_flux<SCHEME, FAST>(P, lim, w_m2, w_m1, w_0, w_p1, nx, ny);
       Gd = Flux{G.f0 * lD, G.f1 * lD, G.f2 * lD, G.f3 * lD};
@@ -162,39 +182,38 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
       const int r = j0 + t;
       if (r >= N) break;
       const Prim w_p2 = prim_of(t + 4);
-      const float vx1 = vxr[t + 1], vy1 = vyr[t + 1];
-      const float vx1r = from_right(vx1), vy1r = from_right(vy1);
+      const R vx1 = vxr[t + 1], vy1 = vyr[t + 1];
+      const R vx1r = from_right(vx1), vy1r = from_right(vy1);
 
       // horizontal face at the left of cell (i, r): stencil i-2..i+1
       const Prim wl = from_left(w_0);
       const Prim wl2 = from_left(wl);
       const Prim wr = from_right(w_0);
-      float nxX, nyX, lL;
+      R nxX, nyX, lL;
       face_normal<FAST>(vx1 - vx0, vy1 - vy0, nxX, nyX, lL);
       const Flux FX = face_flux<SCHEME, FAST>(P, lim, wl2, wl, w_0, wr, nxX, nyX);
       const Flux PX{FX.f0 * lL, FX.f1 * lL, FX.f2 * lL, FX.f3 * lL};
 
       // vertical face at the top of cell (i, r): stencil r-1..r+2
-      float nxY, nyY, lU;
+      R nxY, nyY, lU;
       face_normal<FAST>(vx1 - vx1r, vy1 - vy1r, nxY, nyY, lU);
       const Flux FY = face_flux<SCHEME, FAST>(P, lim, w_m1, w_0, w_p1, w_p2, nxY, nyY);
       const Flux Gu{FY.f0 * lU, FY.f1 * lU, FY.f2 * lU, FY.f3 * lU};
 
       // right face of cell (i, r) is the left face of cell (i+1, r)
       const Flux PR{from_right(PX.f0), from_right(PX.f1), from_right(PX.f2), from_right(PX.f3)};
-      const float lR = from_right(lL);
+      const R lR = from_right(lL);
 
       // quad area (shaders/grid.js:140)
-      const float area =
-          0.5f * fabsf((vx1r - vx0) * (vy1 - vy0r) - (vx1 - vx0r) * (vy1r - vy0));
+      const R area = R(0.5) * fabs((vx1r - vx0) * (vy1 - vy0r) - (vx1 - vx0r) * (vy1r - vy0));
 
       // residual dQ/dt = -(F_R l_R - F_L l_L + G_U l_U - G_D l_D) / A  (ausm.js:434)
-      float res[4];
+      R res[4];
       {
-        const float s0 = ((PR.f0 - PX.f0) + Gu.f0) - Gd.f0;
-        const float s1 = ((PR.f1 - PX.f1) + Gu.f1) - Gd.f1;
-        const float s2 = ((PR.f2 - PX.f2) + Gu.f2) - Gd.f2;
-        const float s3 = ((PR.f3 - PX.f3) + Gu.f3) - Gd.f3;
+        const R s0 = ((PR.f0 - PX.f0) + Gu.f0) - Gd.f0;
+        const R s1 = ((PR.f1 - PX.f1) + Gu.f1) - Gd.f1;
+        const R s2 = ((PR.f2 - PX.f2) + Gu.f2) - Gd.f2;
+        const R s3 = ((PR.f3 - PX.f3) + Gu.f3) - Gd.f3;
         if constexpr (FAST) {
           const float ia = __builtin_amdgcn_rcpf(area);
           res[0] = -s0 * ia;
@@ -211,25 +230,25 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
 
       if (owned) {
         const int o = r * Pp + col;
-        float nq[4];
-        const float cq[4] = {raw[t + 2].r, raw[t + 2].mu, raw[t + 2].mv, raw[t + 2].e};
+        R nq[4];
+        const R cq[4] = {raw[t + 2].r, raw[t + 2].mu, raw[t + 2].mv, raw[t + 2].e};
         if (stage == 0) {
           for (int k = 0; k < 4; ++k) nq[k] = res[k];
         } else if (stage == 1) {
           // Q1 = Qn + dt L(Qn)  (ausm.js:463)
           for (int k = 0; k < 4; ++k) nq[k] = cq[k] + dt * res[k];
         } else {
-          const float qn[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
+          const R qn[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
           if (stage == 2) {
             // Q2 = 3/4 Qn + 1/4 (Q1 + dt L(Q1))  (ausm.js:494)
-            for (int k = 0; k < 4; ++k) nq[k] = 0.75f * qn[k] + 0.25f * (cq[k] + dt * res[k]);
+            for (int k = 0; k < 4; ++k) nq[k] = R(0.75) * qn[k] + R(0.25) * (cq[k] + dt * res[k]);
           } else {
             // Qn+1 = (Qn + 2 (Q2 + dt L(Q2))) / 3  (ausm.js:525)
             if constexpr (FAST) {
               for (int k = 0; k < 4; ++k)
                 nq[k] = fmaf(2.0f, fmaf(dt, res[k], cq[k]), qn[k]) * (1.0f / 3.0f);
             } else {
-              for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + 2.0f * (cq[k] + dt * res[k])) / 3.0f;
+              for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * (cq[k] + dt * res[k])) / R(3);
             }
           }
         }
@@ -239,7 +258,7 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
         a.qout[3 * PS + o] = nq[3];
         if (stage == 3 && a.refresh) {
           // wave speed (|u| + a) * (l_L + l_R + l_U + l_D) / (2 A)  (ausm.js:617-630)
-          float ws;
+          R ws;
           if constexpr (FAST) {
             const float ir = __builtin_amdgcn_rcpf(nq[0]);
             const float vx = nq[1] * ir, vy = nq[2] * ir;
@@ -249,15 +268,15 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
             const float speed = __builtin_amdgcn_sqrtf(vv);
             ws = (speed + c) * (((lL + lR) + lU) + lD) * __builtin_amdgcn_rcpf(2.0f * area);
           } else {
-            const float rho = nq[0];
-            const float vx = nq[1] / rho, vy = nq[2] / rho;
-            const float vv = vx * vx + vy * vy;
-            const float p = (nq[3] - (0.5f * rho) * vv) * (P.gamma - 1.0f);
-            const float T = p / rho;
-            const float c = sqrtf(P.gamma * T);
-            const float speed = sqrtf(vv);
-            const float S = ((lL + lR) + lU) + lD;
-            ws = ((speed + c) * S) / (2.0f * area);
+            const R rho = nq[0];
+            const R vx = nq[1] / rho, vy = nq[2] / rho;
+            const R vv = vx * vx + vy * vy;
+            const R p = (nq[3] - (R(0.5) * rho) * vv) * (P.gamma - R(1));
+            const R T = p / rho;
+            const R c = sqrt(P.gamma * T);
+            const R speed = sqrt(vv);
+            const R S = ((lL + lR) + lU) + lD;
+            ws = ((speed + c) * S) / (R(2) * area);
           }
           ws_acc = nan_max(ws_acc, ws);
         }
@@ -287,28 +306,30 @@ __device__ __forceinline__ float stage_tile(const DevParams& P, const StageArgs&
 
 // dt = min(maxdt, cfl / max wave speed) (ausm.js:457).  On a fresh stage 1
 // every wavefront reduces the per-workgroup maxima of the last refresh.
-__device__ __forceinline__ float stage_dt(const DevParams& P, const StageArgs& a, bool writer) {
+template <typename R>
+__device__ __forceinline__ R stage_dt(const DevParamsT<R>& P, const StageArgsT<R>& a, bool writer) {
   const int lane = threadIdx.x & 63;
-  float maxws;
+  R maxws;
   if (a.stage == 1 && a.fresh) {
-    float m = 0.0f;
+    R m = R(0);
     for (int i = lane; i < a.n_ws_in; i += kWave) m = nan_max(m, a.ws_in[i]);
     maxws = wave_max(m);
     if (writer && threadIdx.x == 0) *a.maxws_cur = maxws;
   } else {
     maxws = *a.maxws_cur;
   }
-  return fminf(P.maxdt, P.cfl / maxws);
+  return fmin(P.maxdt, P.cfl / maxws);
 }
 
 // workgroup max of the wavefronts' wave speeds -> one partial per workgroup tile
-__device__ __forceinline__ void block_ws_partial(float ws_acc, float* out) {
-  __shared__ float s_ws[kWavesPerBlock];
-  const float wm = wave_max(ws_acc);
+template <typename R>
+__device__ __forceinline__ void block_ws_partial(R ws_acc, R* out) {
+  __shared__ R s_ws[kWavesPerBlock];
+  const R wm = wave_max(ws_acc);
   if ((threadIdx.x & 63) == 0) s_ws[threadIdx.x >> 6] = wm;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float m = s_ws[0];
+    R m = s_ws[0];
     for (int w = 1; w < kWavesPerBlock; ++w) m = nan_max(m, s_ws[w]);
     *out = m;
   }

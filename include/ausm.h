@@ -110,6 +110,9 @@ int ausm_grid_from_boundary(ausm_ctx* ctx, const double* body_xy, const double* 
 int ausm_grid_set_vertices(ausm_ctx* ctx, const float* xy);
 int ausm_grid_get_vertices(ausm_ctx* ctx, float* xy);
 int ausm_grid_get_vertices64(ausm_ctx* ctx, double* xy);
+/* FP64 vertices (the FP64 solve uses them as given, the FP32 solve rounded;
+ * ausm_grid_set_vertices widens its FP32 vertices for the FP64 solve) */
+int ausm_grid_set_vertices64(ausm_ctx* ctx, const double* xy);
 int ausm_grid_get_metrics(ausm_ctx* ctx, float* area, float* face_lengths);
 
 /* ---- parameters (Uniforms fields, src/util.js:2-27) --------------------- */
@@ -127,6 +130,17 @@ int ausm_set_cfl(ausm_ctx* ctx, float cfl, float maxdt, int steps_per_dt_refresh
 int ausm_set_math(ausm_ctx* ctx, int math);
 /* rows per wavefront tile of the stage kernel (0 = autotuned) */
 int ausm_set_tile(ausm_ctx* ctx, int rows_per_wave);
+
+/* Arithmetic precision of the solve.  No counterpart in the reference,
+ * whose storage and arithmetic are all f32 (FP32 here).  FP64 runs the same
+ * scheme and pass structure in double (exact arithmetic, FP64 state, the
+ * FP64 grid, the free stream unrounded): the comparison solve that bounds
+ * the error of FP32 results.  Switching carries the current solution over.
+ * In FP64 the FP32 accessors (ausm_get_state, ausm_eval_residual,
+ * ausm_wave_speeds, ausm_step_info's dt and wave speed) return values
+ * rounded to FP32.  Batch contexts are FP32 only. */
+enum ausm_precision { AUSM_PREC_FP32 = 0, AUSM_PREC_FP64 = 1 };
+int ausm_set_precision(ausm_ctx* ctx, int precision);
 
 /* ---- state and stepping (prepareState / render step loop,
  *      main.js:631-655, 742-792) ------------------------------------------ */
@@ -147,6 +161,11 @@ int ausm_wave_speeds(ausm_ctx* ctx, float* ws, float* max_ws);
 int ausm_forces(ausm_ctx* ctx, double ref_length, double* out, double* cp);
 /* FP64 L2 norm of the density residual of the current solution */
 int ausm_residual_norm(ausm_ctx* ctx, double* l2);
+/* FP64 state and residual, layouts as ausm_set_state / ausm_get_state /
+ * ausm_eval_residual (FP64 precision only) */
+int ausm_set_state64(ausm_ctx* ctx, const double* q);
+int ausm_get_state64(ausm_ctx* ctx, double* q);
+int ausm_eval_residual64(ausm_ctx* ctx, double* res);
 
 /* Timing hook, no counterpart in the reference (it times whole passes with
  * timestamp queries, main.js:597-602): launches each of the three stage

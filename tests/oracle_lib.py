@@ -99,6 +99,24 @@ def make_params(mach, aoa_deg, gamma=1.4, rho=1.0, p=None, cfl=0.5, maxdt=1e-4,
     return par.astype(np.float32).astype(dtype)
 
 
+def make_params64(fs, cfl=0.5, maxdt=1e-4, K_p=0.0, K_u=0.75):
+    """FP64 parameter vector of the all-FP64 solve for the free stream
+    fs = (rho, p, vx, vy, gamma) as given (unrounded).  inState is formed in
+    double in the order rho, vx*rho, vy*rho, p/(gamma-1) + 0.5*(vx^2+vy^2)*rho;
+    the solver settings the C ABI passes as FP32 (cfl, maxdt, K_p, K_u) are
+    FP32 values widened."""
+    rho, p, vx, vy, gamma = (float(x) for x in fs)
+    par = np.zeros(P_COUNT, dtype=np.float64)
+    par[P_GAMMA], par[P_INRHO], par[P_INP] = gamma, rho, p
+    par[P_VX], par[P_VY] = vx, vy
+    par[P_IS0:P_IS0 + 4] = [rho, vx * rho, vy * rho,
+                            p / (gamma - 1.0) + 0.5 * (vx * vx + vy * vy) * rho]
+    f = np.float32
+    par[P_MAXDT], par[P_CFL] = float(f(maxdt)), float(f(cfl))
+    par[P_KP], par[P_KU] = float(f(K_p)), float(f(K_u))
+    return par
+
+
 # ----------------------------------------------------------------------------
 # geometry
 # ----------------------------------------------------------------------------
