@@ -1,13 +1,13 @@
 """ausm: MI355X-native explicit O-grid solver step (SLAU2 / SLAU / AUSM+-up,
 MUSCL, TVD-RK3) behind the C ABI of include/ausm.h."""
-from .lib import (AUSMUP, GRID_JACOBI, GRID_SOR, INIT_FREESTREAM, INIT_REST, MATH_EXACT,
-                  MATH_FAST, MINMOD, NOLIMITER, PREC_FP32, PREC_FP64, SLAU, SLAU2, VANALBADA,
-                  VANLEER, AusmError, load)
+from .lib import (AUSMUP, DT_GLOBAL, DT_LOCAL, GRID_JACOBI, GRID_SOR, INIT_FREESTREAM, INIT_REST,
+                  MATH_EXACT, MATH_FAST, MINMOD, NOLIMITER, PREC_FP32, PREC_FP32C, PREC_FP64, SLAU, SLAU2,
+                  VANALBADA, VANLEER, AusmError, load)
 from .batch import Batch
-from .solver import (Solver, circle_boundary, freestream, naca4_boundary, naca4_digits,
-                     rect_boundary)
+from .solver import (Solver, circle_boundary, cst_boundary, freestream, naca4_boundary,
+                     naca4_digits, rect_boundary)
 
-__all__ = ["Solver", "Batch", "AusmError", "load", "naca4_boundary", "naca4_digits", "circle_boundary",
+__all__ = ["Solver", "Batch", "AusmError", "load", "naca4_boundary", "naca4_digits", "cst_boundary", "circle_boundary",
            "rect_boundary", "freestream", "SLAU2", "SLAU", "AUSMUP", "VANALBADA", "MINMOD",
            "VANLEER", "NOLIMITER", "MATH_EXACT", "MATH_FAST", "INIT_REST", "INIT_FREESTREAM",
-           "GRID_JACOBI", "GRID_SOR", "PREC_FP32", "PREC_FP64"]
+           "GRID_JACOBI", "GRID_SOR", "PREC_FP32", "PREC_FP64", "PREC_FP32C", "DT_GLOBAL", "DT_LOCAL"]

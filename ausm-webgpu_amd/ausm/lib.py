@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libausm.so")
 SLAU2, SLAU, AUSMUP = 0, 1, 2
 VANALBADA, MINMOD, VANLEER, NOLIMITER = 0, 1, 2, 3
 MATH_EXACT, MATH_FAST = 0, 1
-PREC_FP32, PREC_FP64 = 0, 1
+PREC_FP32, PREC_FP64, PREC_FP32C = 0, 1, 2
+DT_GLOBAL, DT_LOCAL = 0, 1
 INIT_REST, INIT_FREESTREAM = 0, 1
 GRID_JACOBI, GRID_SOR = 0, 1
 SPACING_POWER, SPACING_COSINE = 0, 1
@@ -48,7 +49,8 @@ class SolveDesc(ctypes.Structure):
                 ("K_p", ctypes.c_float), ("K_u", ctypes.c_float),
                 ("cfl", ctypes.c_float), ("maxdt", ctypes.c_float),
                 ("init_mode", ctypes.c_int), ("max_steps", ctypes.c_int),
-                ("res_drop", ctypes.c_double), ("ref_length", ctypes.c_double)]
+                ("res_drop", ctypes.c_double), ("ref_length", ctypes.c_double),
+                ("dt_mode", ctypes.c_int)]
 
 
 class SolveStatus(ctypes.Structure):
@@ -68,6 +70,7 @@ SIGNATURES = {
     "ausm_ctx_stream": (_i, [_vp, _pp]),
     "ausm_sync": (_i, [_vp]),
     "ausm_boundary_naca4": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _vp]),
+    "ausm_boundary_cst": (_i, [_vp, _vp, _i, _d, _d, _d, _d, _i, _i, _vp]),
     "ausm_boundary_circle": (_i, [_d, _i, _vp]),
     "ausm_boundary_rect": (_i, [_d, _i, _vp]),
     "ausm_grid_from_boundary": (_i, [_vp, _vp, _vp, _i, _i, _d]),
@@ -82,6 +85,7 @@ SIGNATURES = {
     "ausm_set_math": (_i, [_vp, _i]),
     "ausm_set_tile": (_i, [_vp, _i]),
     "ausm_set_precision": (_i, [_vp, _i]),
+    "ausm_set_dt_mode": (_i, [_vp, _i]),
     "ausm_init_state": (_i, [_vp, _i]),
     "ausm_set_state": (_i, [_vp, _vp]),
     "ausm_get_state": (_i, [_vp, _vp]),

@@ -36,6 +36,22 @@ def naca4_boundary(M, code=None, m=0.0, p=0.0, t=0.12, size=0.5, x_offset=0.0, e
     return out
 
 
+def cst_boundary(M, a_upper, a_lower, te_gap=0.0, size=0.5, x_offset=0.0, exponent=1.4,
+                 spacing="power"):
+    """Kulfan class-shape-transformation body of M points, in the point order
+    and placement of naca4_boundary (no counterpart in the reference, which
+    draws NACA 4-digit bodies only).  a_lower is signed (negative below the
+    chord); te_gap = 0 closes the trailing edge."""
+    au, al = f64(a_upper), f64(a_lower)
+    if au.shape != al.shape or au.ndim != 1:
+        raise ValueError("a_upper and a_lower need the same number of coefficients")
+    out = np.zeros((M, 2), dtype=np.float64)
+    sp = L.SPACING_COSINE if spacing == "cosine" else L.SPACING_POWER
+    check(L.load().ausm_boundary_cst(ptr(au), ptr(al), au.size, te_gap, size, x_offset, exponent,
+                                     sp, M, ptr(out)))
+    return out
+
+
 def circle_boundary(M, radius):
     """Circular far field (src/grid.js:232-237)."""
     out = np.zeros((M, 2), dtype=np.float64)
@@ -164,11 +180,19 @@ class Solver:
     def set_tile(self, rows):
         check(self._L.ausm_set_tile(self.h, rows))
 
+    def set_dt_mode(self, mode):
+        """"global" (the reference's single CFL step) or "local" (each cell
+        its own CFL step from Q^n: a steady-state accelerator)."""
+        if isinstance(mode, str):
+            mode = {"global": L.DT_GLOBAL, "local": L.DT_LOCAL}[mode]
+        check(self._L.ausm_set_dt_mode(self.h, mode))
+
     def set_precision(self, prec):
-        """"fp32" (the reference's arithmetic) or "fp64" (the comparison
-        solve); the current solution is carried over."""
+        """"fp32" (the reference's arithmetic), "fp64" (the comparison solve)
+        or "fp32c" (FP32 arithmetic, compensated hi + lo state accumulation);
+        the current solution is carried over."""
         if isinstance(prec, str):
-            prec = {"fp32": L.PREC_FP32, "fp64": L.PREC_FP64}[prec]
+            prec = {"fp32": L.PREC_FP32, "fp64": L.PREC_FP64, "fp32c": L.PREC_FP32C}[prec]
         check(self._L.ausm_set_precision(self.h, prec))
         self._prec = prec
 
@@ -189,6 +213,18 @@ class Solver:
         else:
             a = f32(q).reshape(self.N + 3, self.M, 4)
             check(self._L.ausm_set_state(self.h, ptr(a)))
+
+    def set_state64(self, q):
+        """FP64 / FP32C precision: set the state from float64 (FP32C: split
+        into hi + lo)."""
+        a = f64(q).reshape(self.N + 3, self.M, 4)
+        check(self._L.ausm_set_state64(self.h, ptr(a)))
+
+    def state64(self):
+        """FP64 / FP32C precision: the state as float64 (FP32C: hi + lo)."""
+        out = np.zeros((self.N + 3, self.M, 4), dtype=np.float64)
+        check(self._L.ausm_get_state64(self.h, ptr(out)))
+        return out
 
     def state(self):
         """(N+3) x M x 4 in the reference's texture layout; float64 in the

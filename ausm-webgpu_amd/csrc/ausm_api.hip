@@ -93,6 +93,7 @@ static void widen_settings(ausm_ctx* c) {
 }
 
 static bool fp64(const ausm_ctx* c) { return c->prec == AUSM_PREC_FP64; }
+static bool comp(const ausm_ctx* c) { return c->prec == AUSM_PREC_FP32C; }
 static const double* gx64(const ausm_ctx* c) { return c->g64[c->g_cur]; }
 static const double* gy64(const ausm_ctx* c) { return c->g64[c->g_cur + 1]; }
 
@@ -113,6 +114,7 @@ StageArgs64 stage_args64(ausm_ctx* c, int ty, int stage) {
   a.stage = stage;
   a.limiter = c->limiter;
   a.tiles_x = stage_tiles_x(c->M);
+  a.local_dt = c->local_dt;
   a.qin = c->q64[0];
   a.qn = c->q64[0];
   a.qout = c->q64[1];
@@ -140,6 +142,7 @@ static hipError_t stage_launch(ausm_ctx* c, int ty, int stage, int in, int qn, i
   a.fresh = fresh;
   a.refresh = refresh;
   if (!norm) a.res_out = nullptr;
+  if (comp(c)) return launch_stage_comp(a, c->scheme, c->math, ty, c->stream);
   return launch_stage(a, c->scheme, c->math, ty, c->stream);
 }
 
@@ -153,7 +156,7 @@ static std::map<std::tuple<int, int, int, int, int>, int> g_tune;
 static const int kTileChoices[] = {1, 2, 4, 8};
 
 int autotune_rows(ausm_ctx* c) {
-  const auto key = std::make_tuple(c->M, c->N, c->scheme, fp64(c) ? 2 : c->math, c->limiter);
+  const auto key = std::make_tuple(c->M, c->N, c->scheme, c->prec * 2 + c->math, c->limiter);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
@@ -168,7 +171,7 @@ int autotune_rows(ausm_ctx* c) {
   for (int round = 0; round < 3; ++round) {
     for (int k = 0; k < 4; ++k) {
       const int ty = kTileChoices[k];
-      if (fp64(c) && ty > 4) continue;  // FP64 tiles: 1, 2, 4 rows
+      if (c->prec != AUSM_PREC_FP32 && ty > 4) continue;  // FP64 / compensated: 1, 2, 4 rows
       for (int w = 0; w < 3; ++w) (void)stage_launch(c, ty, 2, 0, 0, 2, false, false, false);
       (void)hipEventRecord(e0, c->stream);
       for (int r = 0; r < 30; ++r) (void)stage_launch(c, ty, 2, 0, 0, 2, false, false, false);
@@ -212,6 +215,9 @@ StageArgs stage_args(ausm_ctx* c, int ty, int stage) {
   a.fresh = 0;
   a.refresh = 0;
   a.tiles_x = stage_tiles_x(c->M);
+  a.local_dt = c->local_dt;
+  a.acc = c->acc;
+  a.lo = c->lo;
   a.qin = c->q[0];
   a.qn = c->q[0];
   a.qout = c->q[1];
@@ -336,6 +342,8 @@ int ausm_ctx_destroy(ausm_ctx* c) {
   for (int k = 0; k < 3; ++k) hipFree(c->q64[k]);
   hipFree(c->ws_part64);
   hipFree(c->stage_buf64);
+  hipFree(c->acc);
+  hipFree(c->lo);
   hipFree(c->stage_buf);
   hipFree(c->seg);
   if (c->pinned) hipHostFree(c->pinned);
@@ -390,6 +398,45 @@ int ausm_boundary_naca4(double m, double p, double t, double size, double x_offs
     const double sgn = upper ? 1.0 : -1.0;
     xy[2 * i] = (xc - 0.5 + sgn * sx) * chord + x_offset;
     xy[2 * i + 1] = (yc + sgn * sy) * chord;
+  }
+  return AUSM_OK;
+}
+
+// Class-shape transformation section (Kulfan): with n = n_coef - 1,
+//   y_u(x) = sqrt(x) (1 - x) sum_i A_u,i C(n,i) x^i (1-x)^(n-i) + x te_gap / 2
+//   y_l(x) = sqrt(x) (1 - x) sum_i A_l,i C(n,i) x^i (1-x)^(n-i) - x te_gap / 2
+// at the chordwise stations of the NACA generator (same point order).
+static double cst_surface(const double* a, int n_coef, double x) {
+  const int n = n_coef - 1;
+  double s = 0.0, binom = 1.0;
+  for (int i = 0; i <= n; ++i) {
+    double xi = 1.0, xr = 1.0;
+    for (int k = 0; k < i; ++k) xi *= x;
+    for (int k = 0; k < n - i; ++k) xr *= 1.0 - x;
+    s += a[i] * binom * xi * xr;
+    binom = binom * (double)(n - i) / (double)(i + 1);
+  }
+  return std::sqrt(x) * (1.0 - x) * s;
+}
+
+int ausm_boundary_cst(const double* a_upper, const double* a_lower, int n_coef, double te_gap,
+                      double size, double x_offset, double exponent, int spacing, int M,
+                      double* xy) {
+  if (!xy || !a_upper || !a_lower || M < 4) return set_error(AUSM_ERR_ARG, "bad CST arguments");
+  if (n_coef < 1 || n_coef > 32) return set_error(AUSM_ERR_ARG, "CST needs 1..32 coefficients per side");
+  const double chord = 2.0 * size;
+  for (int i = 0; i < M; ++i) {
+    const double s = (double)i / ((double)M / 2.0);
+    const bool upper = s <= 1.0;
+    double xc;
+    if (spacing == AUSM_SPACING_COSINE)
+      xc = upper ? 0.5 * (1.0 + std::cos(M_PI * s)) : 0.5 * (1.0 - std::cos(M_PI * (s - 1.0)));
+    else
+      xc = upper ? std::pow(1.0 - s, exponent) : std::pow(s - 1.0, exponent);
+    const double y = upper ? cst_surface(a_upper, n_coef, xc) + xc * (0.5 * te_gap)
+                           : cst_surface(a_lower, n_coef, xc) - xc * (0.5 * te_gap);
+    xy[2 * i] = (xc - 0.5) * chord + x_offset;
+    xy[2 * i + 1] = y * chord;
   }
   return AUSM_OK;
 }
@@ -626,6 +673,15 @@ int ausm_set_math(ausm_ctx* c, int math) {
   return AUSM_OK;
 }
 
+int ausm_set_dt_mode(ausm_ctx* c, int mode) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (mode != AUSM_DT_GLOBAL && mode != AUSM_DT_LOCAL)
+    return set_error(AUSM_ERR_ARG, "unknown dt mode %d", mode);
+  if (mode != c->local_dt) drop_graphs(c);
+  c->local_dt = mode;
+  return AUSM_OK;
+}
+
 int ausm_set_tile(ausm_ctx* c, int rows) {
   if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
   if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8)
@@ -662,6 +718,7 @@ int ausm_init_state(ausm_ctx* c, int mode) {
     CHECK_HIP(launch_fill_state(c->q64[0], c->dpar64, c->M, c->N, c->P, c->PS, mode, c->stream), "init state");
   else
     CHECK_HIP(launch_fill_state(c->q[0], c->dpar, c->M, c->N, c->P, c->PS, mode, c->stream), "init state");
+  if (comp(c)) CHECK_HIP(hipMemsetAsync(c->lo, 0, sizeof(float) * 4 * (size_t)c->PS, c->stream), "init state");
   int r = refresh_ws(c);
   if (r != AUSM_OK) return r;
   CHECK_HIP(hipStreamSynchronize(c->stream), "init state");
@@ -682,6 +739,7 @@ int ausm_set_state(ausm_ctx* c, const float* q) {
     CHECK_HIP(launch_convert_planes(c->q[1], c->q64[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
   } else {
     CHECK_HIP(launch_aos_to_soa(c->stage_buf, c->q[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+    if (comp(c)) CHECK_HIP(hipMemsetAsync(c->lo, 0, sizeof(float) * 4 * (size_t)c->PS, c->stream), "state upload");
   }
   int r = refresh_ws(c);
   if (r != AUSM_OK) return r;
@@ -710,44 +768,72 @@ int ausm_get_state(ausm_ctx* c, float* q) {
 
 static int current_rows(ausm_ctx* c) {
   const int t = c->tile ? c->tile : autotune_rows(c);
-  return fp64(c) ? std::min(t, 4) : t;  // FP64 tiles: 1, 2, 4 rows
+  return c->prec != AUSM_PREC_FP32 ? std::min(t, 4) : t;  // FP64 / compensated: 1, 2, 4 rows
+}
+
+static int ensure_fp64_buffers(ausm_ctx* c) {
+  if (c->q64[0]) return AUSM_OK;
+  const size_t nq = sizeof(double) * 4 * (size_t)c->PS;
+  bool ok = true;
+  for (int k = 0; k < 3 && ok; ++k) ok = hipMalloc(&c->q64[k], nq) == hipSuccess;
+  ok = ok && hipMalloc(&c->ws_part64, sizeof(double) * c->max_parts) == hipSuccess;
+  ok = ok && hipMalloc(&c->stage_buf64, sizeof(double) * 4 * (size_t)c->M * (c->N + 3)) == hipSuccess;
+  if (!ok) {
+    for (int k = 0; k < 3; ++k) {
+      hipFree(c->q64[k]);
+      c->q64[k] = nullptr;
+    }
+    hipFree(c->ws_part64);
+    hipFree(c->stage_buf64);
+    c->ws_part64 = nullptr;
+    c->stage_buf64 = nullptr;
+    return set_error(AUSM_ERR_NOMEM, "device allocation failed for the FP64 state");
+  }
+  return AUSM_OK;
+}
+
+static int ensure_comp_buffers(ausm_ctx* c) {
+  if (c->acc) return AUSM_OK;
+  const size_t nq = sizeof(float) * 4 * (size_t)c->PS;
+  if (hipMalloc(&c->acc, nq) != hipSuccess || hipMalloc(&c->lo, nq) != hipSuccess) {
+    hipFree(c->acc);
+    hipFree(c->lo);
+    c->acc = c->lo = nullptr;
+    return set_error(AUSM_ERR_NOMEM, "device allocation failed for the compensated state");
+  }
+  return AUSM_OK;
 }
 
 int ausm_set_precision(ausm_ctx* c, int precision) {
   if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
-  if (precision != AUSM_PREC_FP32 && precision != AUSM_PREC_FP64)
+  if (precision != AUSM_PREC_FP32 && precision != AUSM_PREC_FP64 && precision != AUSM_PREC_FP32C)
     return set_error(AUSM_ERR_ARG, "unknown precision %d", precision);
   if (precision == c->prec) return AUSM_OK;
   hipSetDevice(c->device);
-  if (precision == AUSM_PREC_FP64 && !c->q64[0]) {
-    const size_t nq = sizeof(double) * 4 * (size_t)c->PS;
-    bool ok = true;
-    for (int k = 0; k < 3 && ok; ++k) ok = hipMalloc(&c->q64[k], nq) == hipSuccess;
-    ok = ok && hipMalloc(&c->ws_part64, sizeof(double) * c->max_parts) == hipSuccess;
-    ok = ok && hipMalloc(&c->stage_buf64, sizeof(double) * 4 * (size_t)c->M * (c->N + 3)) == hipSuccess;
-    if (!ok) {
-      for (int k = 0; k < 3; ++k) {
-        hipFree(c->q64[k]);
-        c->q64[k] = nullptr;
-      }
-      hipFree(c->ws_part64);
-      hipFree(c->stage_buf64);
-      c->ws_part64 = nullptr;
-      c->stage_buf64 = nullptr;
-      return set_error(AUSM_ERR_NOMEM, "device allocation failed for the FP64 state");
-    }
-  }
+  int r = AUSM_OK;
+  if (precision == AUSM_PREC_FP64) r = ensure_fp64_buffers(c);
+  if (r == AUSM_OK && precision == AUSM_PREC_FP32C) r = ensure_comp_buffers(c);
+  if (r != AUSM_OK) return r;
   drop_graphs(c);
   // carry the current solution over to the new precision
   if (c->has_state) {
-    if (precision == AUSM_PREC_FP64)
-      CHECK_HIP(launch_convert_planes(c->q[0], c->q64[0], c->M, c->N, c->P, c->PS, c->stream), "precision switch");
-    else
-      CHECK_HIP(launch_convert_planes(c->q64[0], c->q[0], c->M, c->N, c->P, c->PS, c->stream), "precision switch");
+    const int M = c->M, N = c->N, P = c->P, PS = c->PS;
+    const int from = c->prec, to = precision;
+    if (to == AUSM_PREC_FP32C)
+      CHECK_HIP(hipMemsetAsync(c->lo, 0, sizeof(float) * 4 * (size_t)PS, c->stream), "precision switch");
+    if (from == AUSM_PREC_FP64 && to == AUSM_PREC_FP32C)
+      CHECK_HIP(launch_f64_to_comp(c->q64[0], c->q[0], c->lo, M, N, P, PS, c->stream), "precision switch");
+    else if (from == AUSM_PREC_FP64)
+      CHECK_HIP(launch_convert_planes(c->q64[0], c->q[0], M, N, P, PS, c->stream), "precision switch");
+    else if (to == AUSM_PREC_FP64 && from == AUSM_PREC_FP32C)
+      CHECK_HIP(launch_comp_to_f64(c->q[0], c->lo, c->q64[0], M, N, P, PS, c->stream), "precision switch");
+    else if (to == AUSM_PREC_FP64)
+      CHECK_HIP(launch_convert_planes(c->q[0], c->q64[0], M, N, P, PS, c->stream), "precision switch");
+    // FP32 <-> FP32C: hi is the FP32 state (round(hi + lo) = hi)
   }
   c->prec = precision;
   if (c->has_state) {
-    const int r = refresh_ws(c);
+    r = refresh_ws(c);
     if (r != AUSM_OK) return r;
     c->since_refresh = 0;
   }
@@ -757,12 +843,17 @@ int ausm_set_precision(ausm_ctx* c, int precision) {
 
 int ausm_set_state64(ausm_ctx* c, const double* q) {
   if (!c || !q) return set_error(AUSM_ERR_ARG, "NULL argument");
-  if (!fp64(c)) return set_error(AUSM_ERR_ORDER, "FP64 state needs FP64 precision");
+  if (!fp64(c) && !comp(c))
+    return set_error(AUSM_ERR_ORDER, "FP64 state needs FP64 or compensated precision");
   if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "set a grid before the state");
   hipSetDevice(c->device);
+  const int rb = ensure_fp64_buffers(c);
+  if (rb != AUSM_OK) return rb;
   const size_t n = 4 * (size_t)c->M * (c->N + 3);
   CHECK_HIP(hipMemcpyAsync(c->stage_buf64, q, sizeof(double) * n, hipMemcpyHostToDevice, c->stream), "state upload");
   CHECK_HIP(launch_aos_to_soa(c->stage_buf64, c->q64[0], c->M, c->N, c->P, c->PS, c->stream), "state upload");
+  // compensated: split into hi + lo
+  if (comp(c)) CHECK_HIP(launch_f64_to_comp(c->q64[0], c->q[0], c->lo, c->M, c->N, c->P, c->PS, c->stream), "state upload");
   const int r = refresh_ws(c);
   if (r != AUSM_OK) return r;
   CHECK_HIP(hipStreamSynchronize(c->stream), "state upload");
@@ -774,10 +865,15 @@ int ausm_set_state64(ausm_ctx* c, const double* q) {
 
 int ausm_get_state64(ausm_ctx* c, double* q) {
   if (!c || !q) return set_error(AUSM_ERR_ARG, "NULL argument");
-  if (!fp64(c)) return set_error(AUSM_ERR_ORDER, "FP64 state needs FP64 precision");
+  if (!fp64(c) && !comp(c))
+    return set_error(AUSM_ERR_ORDER, "FP64 state needs FP64 or compensated precision");
   if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
   hipSetDevice(c->device);
+  const int rb = ensure_fp64_buffers(c);
+  if (rb != AUSM_OK) return rb;
   const size_t n = 4 * (size_t)c->M * (c->N + 3);
+  // compensated: hi + lo (exact in FP64); the ghost rows are formed in FP64
+  if (comp(c)) CHECK_HIP(launch_comp_to_f64(c->q[0], c->lo, c->q64[0], c->M, c->N, c->P, c->PS, c->stream), "state download");
   CHECK_HIP(launch_soa_to_aos(c->q64[0], gx64(c), gy64(c), c->dpar64, c->M, c->N, c->P, c->PS, c->stage_buf64, c->stream), "state download");
   CHECK_HIP(hipMemcpyAsync(q, c->stage_buf64, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream), "state download");
   CHECK_HIP(hipStreamSynchronize(c->stream), "state download");
@@ -983,6 +1079,14 @@ int ausm_profile_stages(ausm_ctx* c, int reps, float* ms3) {
   hipEvent_t e0, e1;
   CHECK_HIP(hipEventCreate(&e0), "event");
   CHECK_HIP(hipEventCreate(&e1), "event");
+  // compensated stage 3 updates lo in place: point it at the scratch
+  // accumulator while timing
+  struct LoRestore {
+    ausm_ctx* c;
+    float* keep;
+    ~LoRestore() { c->lo = keep; }
+  } lo_restore{c, c->lo};
+  c->lo = c->acc;
   for (int st = 1; st <= 3; ++st) {
     // scratch outputs only; the residual norm is formed on check steps only
     const int in = st == 1 ? 0 : (st == 2 ? 1 : 2);

@@ -317,6 +317,29 @@ __global__ void fill_state_kernel(R* q, const DevParamsT<R>* par, int M, int N, 
   }
 }
 
+// compensated FP32 pair (hi + lo) <-> FP64, interior rows, 4 planes
+__global__ void comp_to_f64_kernel(const float* hi, const float* lo, double* q, int M, int N, int P,
+                                   int PS) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  const int o = y * P + x;
+  for (int k = 0; k < 4; ++k) q[k * PS + o] = (double)hi[k * PS + o] + (double)lo[k * PS + o];
+}
+__global__ void f64_to_comp_kernel(const double* q, float* hi, float* lo, int M, int N, int P,
+                                   int PS) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  const int o = y * P + x;
+  for (int k = 0; k < 4; ++k) {
+    const double v = q[k * PS + o];
+    const float h = (float)v;
+    hi[k * PS + o] = h;
+    lo[k * PS + o] = (float)(v - (double)h);
+  }
+}
+
 // state planes between precisions (interior rows, 4 planes)
 template <typename A, typename B>
 __global__ void convert_planes_kernel(const A* a, B* b, int M, int N, int P, int PS) {
@@ -406,6 +429,16 @@ hipError_t launch_fill_state(R* q, const DevParamsT<R>* par, int M, int N, int P
                              int freestream, hipStream_t s) {
   hipLaunchKernelGGL(fill_state_kernel<R>, rows_grid(M, N), dim3(256), 0, s, q, par, M, N, P, PS,
                      freestream);
+  return hipGetLastError();
+}
+hipError_t launch_comp_to_f64(const float* hi, const float* lo, double* q, int M, int N, int P,
+                              int PS, hipStream_t s) {
+  hipLaunchKernelGGL(comp_to_f64_kernel, rows_grid(M, N), dim3(256), 0, s, hi, lo, q, M, N, P, PS);
+  return hipGetLastError();
+}
+hipError_t launch_f64_to_comp(const double* q, float* hi, float* lo, int M, int N, int P, int PS,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(f64_to_comp_kernel, rows_grid(M, N), dim3(256), 0, s, q, hi, lo, M, N, P, PS);
   return hipGetLastError();
 }
 template <typename A, typename B>

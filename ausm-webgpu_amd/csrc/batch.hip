@@ -269,6 +269,7 @@ static int solve_norm(ausm_batch* b, Slot* s, double* out) {
   a.fresh = 1;
   a.refresh = 0;
   a.tiles_x = s->dev.tiles_x;
+  a.local_dt = s->d.dt_mode == AUSM_DT_LOCAL;
   if (launch_stage(a, s->d.scheme, b->math, 2, b->stream) != hipSuccess ||
       launch_reduce_sum_f64(s->dev.res_part, s->d.N * s->dev.tiles_x, b->d_tmpd, b->stream) !=
           hipSuccess ||
@@ -319,6 +320,7 @@ static int admit(ausm_batch* b) {
     D.limiter = s->d.limiter;
     D.n_ws = L.nws_max;
     D.active = 1;
+    D.local_dt = s->d.dt_mode == AUSM_DT_LOCAL;
     make_freestream(D.par, s->d.rho, s->d.p, s->d.vx, s->d.vy, s->d.gamma);
     D.par.Kp = s->d.K_p;
     D.par.Ku = s->d.K_u;
@@ -594,6 +596,8 @@ int ausm_batch_submit(ausm_batch* b, const ausm_solve_desc* d, uint64_t* id) {
     return set_error(AUSM_ERR_ARG, "bad flow or run settings");
   if (d->init_mode != AUSM_INIT_REST && d->init_mode != AUSM_INIT_FREESTREAM)
     return set_error(AUSM_ERR_ARG, "bad init mode");
+  if (d->dt_mode != AUSM_DT_GLOBAL && d->dt_mode != AUSM_DT_LOCAL)
+    return set_error(AUSM_ERR_ARG, "bad dt mode");
   const Layout L = layout_of(d->M, d->N);
   if (L.total > b->arena.size())
     return set_error(AUSM_ERR_FULL, "a %d x %d solve needs %zu bytes, more than the whole budget", d->M, d->N, L.total);

@@ -37,6 +37,7 @@ __device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, in
   a.PS = S.PS;
   a.limiter = S.limiter;
   a.tiles_x = S.tiles_x;
+  a.local_dt = S.local_dt;
   a.fresh = 0;
   a.refresh = refresh;
   if (stage == 1) {

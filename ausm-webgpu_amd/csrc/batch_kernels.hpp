@@ -20,7 +20,7 @@ struct SolveDev {
   int scheme, limiter;
   int n_ws;    // workgroup tiles of this solve = entries of ws_part
   int active;  // 0: slot not stepped
-  int pad;
+  int local_dt;  // 1: local time stepping
 };
 
 struct BatchItem {

@@ -41,6 +41,7 @@ struct ausm_ctx {
   bool ws_fresh = false;
   bool has_grid = false, has_state = false;
   int scheme = 0, limiter = 0, math = 0, refresh = 1, tile = 0;
+  int local_dt = 0;  // AUSM_DT_LOCAL: each cell steps with its own CFL dt
   bool use_graphs = true;
   long long steps = 0;
   int since_refresh = 0;  // steps since the wave speed was last refreshed
@@ -52,6 +53,9 @@ struct ausm_ctx {
   double* q64[3] = {nullptr, nullptr, nullptr};   // Q^n, Q1, Q2 in FP64
   double* ws_part64 = nullptr;
   double* stage_buf64 = nullptr;                  // AoS staging, (N+3) x M x 4 doubles
+  // compensated FP32 state (AUSM_PREC_FP32C): q[0] holds hi
+  float* acc = nullptr;  // increment accumulator planes
+  float* lo = nullptr;   // low part of the solution
 };
 
 namespace ausm {

@@ -34,6 +34,11 @@ struct StageArgsT {
   int fresh;    // stage 1: reduce ws_in into maxws_cur
   int refresh;  // stage 3: write ws_out
   int tiles_x;  // wavefront tiles around i
+  int local_dt; // 1: each cell steps with its own CFL dt from Q^n (local time stepping)
+  // compensated-state stages only: the increment accumulator (K1, then
+  // K1 + K2) and the low part of the solution, both SoA planes like qout
+  R* acc;
+  R* lo;
 };
 using StageArgs = StageArgsT<float>;
 using StageArgs64 = StageArgsT<double>;
@@ -43,5 +48,7 @@ int stage_tiles_x(int M);
 hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s);
 // all-FP64 stage (exact arithmetic only); ty in {1, 2, 4}
 hipError_t launch_stage64(const StageArgs64& a, int scheme, int ty, hipStream_t s);
+// compensated-state FP32 stage; ty in {1, 2, 4}
+hipError_t launch_stage_comp(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s);
 
 }  // namespace ausm

@@ -48,6 +48,18 @@ __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
+// Compensated-state FP32 variant (the COMP stages of stage_tile).
+template <int SCHEME, bool FAST, int TY>
+__global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
+  const DevParams P = *a.par;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = tile % a.tiles_x;
+  const int tyb = tile / a.tiles_x;
+  const float dt = stage_dt(P, a, blockIdx.x == 0);
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true>(P, a, tx, tyb, dt);
+  if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
+}
+
 int stage_tiles_x(int M) { return (M + kOwn - 1) / kOwn; }
 
 int stage_blocks(int M, int N, int ty) {
@@ -81,6 +93,30 @@ hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStr
       case SCHEME_SLAU: return launch_ty<SCHEME_SLAU, false>(a, ty, s);
       case SCHEME_AUSMUP: return launch_ty<SCHEME_AUSMUP, false>(a, ty, s);
     }
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int SCHEME, bool FAST>
+static hipError_t launch_ty_comp(const StageArgs& a, int ty, hipStream_t s) {
+  const int nb = stage_blocks(a.M, a.N, ty);
+  switch (ty) {
+    case 1: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 2>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 4>), dim3(nb), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_comp(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s) {
+  switch (scheme * 2 + (fast ? 1 : 0)) {
+    case SCHEME_SLAU2 * 2: return launch_ty_comp<SCHEME_SLAU2, false>(a, ty, s);
+    case SCHEME_SLAU2 * 2 + 1: return launch_ty_comp<SCHEME_SLAU2, true>(a, ty, s);
+    case SCHEME_SLAU * 2: return launch_ty_comp<SCHEME_SLAU, false>(a, ty, s);
+    case SCHEME_SLAU * 2 + 1: return launch_ty_comp<SCHEME_SLAU, true>(a, ty, s);
+    case SCHEME_AUSMUP * 2: return launch_ty_comp<SCHEME_AUSMUP, false>(a, ty, s);
+    case SCHEME_AUSMUP * 2 + 1: return launch_ty_comp<SCHEME_AUSMUP, true>(a, ty, s);
   }
   return hipErrorInvalidValue;
 }

@@ -70,12 +70,44 @@ __device__ __forceinline__ void face_normal(R dx, R dy, R& nx, R& ny, R& len) {
   else normal_exact(dx, dy, nx, ny, len);
 }
 
+// CFL wave speed of one cell, (|u| + c) * (l_L + l_R + l_U + l_D) / (2 A)
+// (ausm.js:617-630); S is the sum of the four face lengths.
+template <bool FAST, typename R>
+__device__ __forceinline__ R cell_wave_speed(const DevParamsT<R>& P, R rho, R mu, R mv, R e, R S,
+                                             R area) {
+  if constexpr (FAST) {
+    const float ir = __builtin_amdgcn_rcpf(rho);
+    const float vx = mu * ir, vy = mv * ir;
+    const float vv = fmaf(vx, vx, vy * vy);
+    const float p = fmaf(-0.5f * rho, vv, e) * P.gm1;
+    const float c = __builtin_amdgcn_sqrtf(P.gamma * p * ir);
+    const float speed = __builtin_amdgcn_sqrtf(vv);
+    return (speed + c) * S * __builtin_amdgcn_rcpf(2.0f * area);
+  } else {
+    const R vx = mu / rho, vy = mv / rho;
+    const R vv = vx * vx + vy * vy;
+    const R p = (e - (R(0.5) * rho) * vv) * (P.gamma - R(1));
+    const R T = p / rho;
+    const R c = sqrt(P.gamma * T);
+    const R speed = sqrt(vv);
+    return ((speed + c) * S) / (R(2) * area);
+  }
+}
+
 // One wavefront's share of a workgroup tile: the strip of columns
 // [60*tx, 60*tx+60) and rows [(4*tyb + wave)*TY, ... + TY) of one grid.
 // Returns the wavefront's max wave speed of the new state (stage 3 with
 // refresh; 0 otherwise).  All 64 lanes must call it together.  R = double
 // is the all-FP64 variant of the exact arithmetic.
-template <int SCHEME, bool FAST, int TY, typename R = float>
+//
+// COMP: compensated-state stages.  The same TVD-RK3 in increment form,
+//   K_s = dt L(Q_{s-1}),  Q1 = Q^n + K1,  Q2 = Q^n + (K1 + K2)/4,
+//   Q^{n+1} = Q^n + (K1 + K2 + 4 K3)/6,
+// with Q^n held as hi + lo (qn / qout = hi, a.lo = lo) and the last
+// increment added by an error-free TwoSum, so that increments below half an
+// ulp of hi accumulate in lo instead of being rounded away.  a.acc carries
+// K1, then K1 + K2, from stage to stage at the cell.
+template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false>
 __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
                                         int tyb, R dt) {
   static_assert(!FAST || sizeof(R) == 4, "the fast arithmetic is FP32 only");
@@ -124,6 +156,19 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       for (int k = 0; k < TY; ++k) {
         const int o = min(j0 + k, N - 1) * Pp + col;
         qnr[k] = Cons{a.qn[o], a.qn[PS + o], a.qn[2 * PS + o], a.qn[3 * PS + o]};
+      }
+    }
+    // compensated stages: the increment accumulator (stages 2, 3) and the
+    // low part of Q^n (stage 3) at the tile's cells
+    Cons accr[COMP ? TY : 1], lor[COMP ? TY : 1];
+    if constexpr (COMP) {
+      if (stage >= 2) {
+#pragma unroll
+        for (int k = 0; k < TY; ++k) {
+          const int o = min(j0 + k, N - 1) * Pp + col;
+          accr[k] = Cons{a.acc[o], a.acc[PS + o], a.acc[2 * PS + o], a.acc[3 * PS + o]};
+          if (stage == 3) lor[k] = Cons{a.lo[o], a.lo[PS + o], a.lo[2 * PS + o], a.lo[3 * PS + o]};
+        }
       }
     }
     // ghost rows, formed once per wavefront when its window reaches them:
@@ -232,23 +277,74 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         const int o = r * Pp + col;
         R nq[4];
         const R cq[4] = {raw[t + 2].r, raw[t + 2].mu, raw[t + 2].mv, raw[t + 2].e};
+        R dt_c = dt;
+        if (a.local_dt && stage >= 1) {
+          // local time stepping: the cell's own CFL step from Q^n, the same
+          // in all three stages
+          const Cons& q0 = stage == 1 ? raw[t + 2] : qnr[t];
+          const R ws0 = cell_wave_speed<FAST>(P, q0.r, q0.mu, q0.mv, q0.e, ((lL + lR) + lU) + lD,
+                                              area);
+          if constexpr (FAST) dt_c = fminf(P.maxdt, P.cfl * __builtin_amdgcn_rcpf(ws0));
+          else dt_c = fmin(P.maxdt, P.cfl / ws0);
+        }
         if (stage == 0) {
           for (int k = 0; k < 4; ++k) nq[k] = res[k];
+        } else if constexpr (COMP) {
+          R an[4];
+          if (stage == 1) {
+            // K1 = dt L(Q^n);  Q1 = hi + K1
+            for (int k = 0; k < 4; ++k) {
+              an[k] = dt_c * res[k];
+              nq[k] = cq[k] + an[k];
+            }
+          } else {
+            const R qh[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
+            const R ac[4] = {accr[t].r, accr[t].mu, accr[t].mv, accr[t].e};
+            if (stage == 2) {
+              // K1 + K2;  Q2 = hi + (K1 + K2)/4
+              for (int k = 0; k < 4; ++k) {
+                an[k] = ac[k] + dt_c * res[k];
+                nq[k] = qh[k] + R(0.25) * an[k];
+              }
+            } else {
+              // D = (K1 + K2 + 4 K3)/6 added to hi + lo by TwoSum
+              const R lw[4] = {lor[t].r, lor[t].mu, lor[t].mv, lor[t].e};
+              for (int k = 0; k < 4; ++k) {
+                const R K = dt_c * res[k];
+                R D;
+                if constexpr (FAST) D = fmaf(4.0f, K, ac[k]) * (1.0f / 6.0f);
+                else D = (ac[k] + R(4) * K) / R(6);
+                const R sm = qh[k] + D;
+                const R b = sm - qh[k];
+                const R e = (qh[k] - (sm - b)) + (D - b);
+                const R tt = lw[k] + e;
+                const R hn = sm + tt;
+                const R b2 = hn - sm;
+                an[k] = (sm - (hn - b2)) + (tt - b2);  // new lo
+                nq[k] = hn;
+              }
+            }
+          }
+          R* dst = stage == 3 ? a.lo : a.acc;
+          dst[o] = an[0];
+          dst[PS + o] = an[1];
+          dst[2 * PS + o] = an[2];
+          dst[3 * PS + o] = an[3];
         } else if (stage == 1) {
           // Q1 = Qn + dt L(Qn)  (ausm.js:463)
-          for (int k = 0; k < 4; ++k) nq[k] = cq[k] + dt * res[k];
+          for (int k = 0; k < 4; ++k) nq[k] = cq[k] + dt_c * res[k];
         } else {
           const R qn[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
           if (stage == 2) {
             // Q2 = 3/4 Qn + 1/4 (Q1 + dt L(Q1))  (ausm.js:494)
-            for (int k = 0; k < 4; ++k) nq[k] = R(0.75) * qn[k] + R(0.25) * (cq[k] + dt * res[k]);
+            for (int k = 0; k < 4; ++k) nq[k] = R(0.75) * qn[k] + R(0.25) * (cq[k] + dt_c * res[k]);
           } else {
             // Qn+1 = (Qn + 2 (Q2 + dt L(Q2))) / 3  (ausm.js:525)
             if constexpr (FAST) {
               for (int k = 0; k < 4; ++k)
-                nq[k] = fmaf(2.0f, fmaf(dt, res[k], cq[k]), qn[k]) * (1.0f / 3.0f);
+                nq[k] = fmaf(2.0f, fmaf(dt_c, res[k], cq[k]), qn[k]) * (1.0f / 3.0f);
             } else {
-              for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * (cq[k] + dt * res[k])) / R(3);
+              for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * (cq[k] + dt_c * res[k])) / R(3);
             }
           }
         }
@@ -257,27 +353,8 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         a.qout[2 * PS + o] = nq[2];
         a.qout[3 * PS + o] = nq[3];
         if (stage == 3 && a.refresh) {
-          // wave speed (|u| + a) * (l_L + l_R + l_U + l_D) / (2 A)  (ausm.js:617-630)
-          R ws;
-          if constexpr (FAST) {
-            const float ir = __builtin_amdgcn_rcpf(nq[0]);
-            const float vx = nq[1] * ir, vy = nq[2] * ir;
-            const float vv = fmaf(vx, vx, vy * vy);
-            const float p = fmaf(-0.5f * nq[0], vv, nq[3]) * P.gm1;
-            const float c = __builtin_amdgcn_sqrtf(P.gamma * p * ir);
-            const float speed = __builtin_amdgcn_sqrtf(vv);
-            ws = (speed + c) * (((lL + lR) + lU) + lD) * __builtin_amdgcn_rcpf(2.0f * area);
-          } else {
-            const R rho = nq[0];
-            const R vx = nq[1] / rho, vy = nq[2] / rho;
-            const R vv = vx * vx + vy * vy;
-            const R p = (nq[3] - (R(0.5) * rho) * vv) * (P.gamma - R(1));
-            const R T = p / rho;
-            const R c = sqrt(P.gamma * T);
-            const R speed = sqrt(vv);
-            const R S = ((lL + lR) + lU) + lD;
-            ws = ((speed + c) * S) / (R(2) * area);
-          }
+          const R ws = cell_wave_speed<FAST>(P, nq[0], nq[1], nq[2], nq[3],
+                                             ((lL + lR) + lU) + lD, area);
           ws_acc = nan_max(ws_acc, ws);
         }
       }

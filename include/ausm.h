@@ -99,6 +99,15 @@ int ausm_sync(ausm_ctx* ctx);
  * centred at x_offset; M points CCW from the trailing edge. */
 int ausm_boundary_naca4(double m, double p, double t, double size, double x_offset,
                         double exponent, int spacing, int M, double* xy);
+/* Class-shape transformation (Kulfan CST) section; no counterpart in the
+ * reference, which generates NACA 4-digit bodies only.  n_coef Bernstein
+ * coefficients per side (a_lower signed: negative below the chord), class
+ * function sqrt(x)(1-x) (round nose of radius a_upper[0]^2/2), trailing-edge
+ * gap te_gap (0 = closed); chord stations, point order and placement as
+ * ausm_boundary_naca4. */
+int ausm_boundary_cst(const double* a_upper, const double* a_lower, int n_coef, double te_gap,
+                      double size, double x_offset, double exponent, int spacing, int M,
+                      double* xy);
 int ausm_boundary_circle(double radius, int M, double* xy);
 int ausm_boundary_rect(double length_ratio, int M, double* xy);
 
@@ -128,6 +137,15 @@ int ausm_set_scheme(ausm_ctx* ctx, int scheme, int limiter, float K_p, float K_u
  * each frame). */
 int ausm_set_cfl(ausm_ctx* ctx, float cfl, float maxdt, int steps_per_dt_refresh);
 int ausm_set_math(ausm_ctx* ctx, int math);
+/* Time step.  GLOBAL is the reference's: one dt = min(maxdt, cfl / max
+ * wave speed) for every cell (main.js:789-791, ausm.js:457).  LOCAL has no
+ * counterpart in the reference: each cell steps with its own
+ * min(maxdt, cfl / wave speed) from Q^n, the same in all three RK stages --
+ * the usual convergence accelerator for steady solves (the spatial operator
+ * and hence the steady state are unchanged; the transient is not
+ * time-accurate).  ausm_step_info.dt still reports the global dt. */
+enum ausm_dt_mode { AUSM_DT_GLOBAL = 0, AUSM_DT_LOCAL = 1 };
+int ausm_set_dt_mode(ausm_ctx* ctx, int mode);
 /* rows per wavefront tile of the stage kernel (0 = autotuned) */
 int ausm_set_tile(ausm_ctx* ctx, int rows_per_wave);
 
@@ -135,11 +153,17 @@ int ausm_set_tile(ausm_ctx* ctx, int rows_per_wave);
  * whose storage and arithmetic are all f32 (FP32 here).  FP64 runs the same
  * scheme and pass structure in double (exact arithmetic, FP64 state, the
  * FP64 grid, the free stream unrounded): the comparison solve that bounds
- * the error of FP32 results.  Switching carries the current solution over.
- * In FP64 the FP32 accessors (ausm_get_state, ausm_eval_residual,
- * ausm_wave_speeds, ausm_step_info's dt and wave speed) return values
- * rounded to FP32.  Batch contexts are FP32 only. */
-enum ausm_precision { AUSM_PREC_FP32 = 0, AUSM_PREC_FP64 = 1 };
+ * the error of FP32 results.  FP32C keeps every flux and stage operation in
+ * FP32 but holds the solution as an unevaluated pair hi + lo and adds each
+ * step's increment with an error-free TwoSum (the same TVD-RK3 in increment
+ * form, Q^{n+1} = Q^n + (K1 + K2 + 4 K3)/6): plain FP32 accumulation
+ * stagnates near a steady state once dt*L(Q) falls below half an ulp of Q,
+ * FP32C does not.  Switching carries the current solution over.  In FP64
+ * the FP32 accessors (ausm_get_state, ausm_eval_residual, ausm_wave_speeds,
+ * ausm_step_info's dt and wave speed) return values rounded to FP32; in
+ * FP32C they return hi, and ausm_get_state64 returns hi + lo.  Batch
+ * contexts are FP32 only. */
+enum ausm_precision { AUSM_PREC_FP32 = 0, AUSM_PREC_FP64 = 1, AUSM_PREC_FP32C = 2 };
 int ausm_set_precision(ausm_ctx* ctx, int precision);
 
 /* ---- state and stepping (prepareState / render step loop,
@@ -162,7 +186,9 @@ int ausm_forces(ausm_ctx* ctx, double ref_length, double* out, double* cp);
 /* FP64 L2 norm of the density residual of the current solution */
 int ausm_residual_norm(ausm_ctx* ctx, double* l2);
 /* FP64 state and residual, layouts as ausm_set_state / ausm_get_state /
- * ausm_eval_residual (FP64 precision only) */
+ * ausm_eval_residual.  The state functions work in FP64 and FP32C precision
+ * (FP32C: the value is split into / read back as hi + lo), the residual in
+ * FP64 only. */
 int ausm_set_state64(ausm_ctx* ctx, const double* q);
 int ausm_get_state64(ausm_ctx* ctx, double* q);
 int ausm_eval_residual64(ausm_ctx* ctx, double* res);
@@ -190,6 +216,7 @@ typedef struct ausm_solve_desc {
   int max_steps;         /* stop after this many steps */
   double res_drop;       /* converged when res_l2 <= res_drop * first res_l2 (0 = never) */
   double ref_length;     /* for Cl/Cd */
+  int dt_mode;           /* AUSM_DT_GLOBAL or AUSM_DT_LOCAL (ausm_set_dt_mode) */
 } ausm_solve_desc;
 
 typedef struct ausm_solve_status {

@@ -398,20 +398,26 @@ R FN(dt_from_ws)(const R* par, R maxws) { return rmin(par[P_MAXDT], par[P_CFL] /
  * k=1: out = s0 + dt*res
  * k=2: out = 0.75*s0 + 0.25*(s1 + dt*res)
  * k=3: out = (s0 + 2*(s1 + dt*res)) / 3                                  */
-void FN(stage_pass)(int k, int M, int N, R dt, const R* res, const R* s0, const R* s1, R* out) {
+static void stage_cells(int k, int M, int N, R dt, const R* dtc, const R* res, const R* s0,
+                        const R* s1, R* out) {
 #pragma omp parallel for schedule(static)
   for (int y = 0; y < N; ++y)
     for (int x = 0; x < M; ++x) {
       const size_t ci = ((size_t)(y + 1) * M + x) * 4;
       const R* rr = res + ((size_t)y * M + x) * 4;
+      const R h = dtc ? dtc[(size_t)y * M + x] : dt;
       for (int c = 0; c < 4; ++c) {
         R v;
-        if (k == 1) v = s0[ci + c] + dt * rr[c];
-        else if (k == 2) v = (R)0.75 * s0[ci + c] + (R)0.25 * (s1[ci + c] + dt * rr[c]);
-        else v = (s0[ci + c] + (R)2 * (s1[ci + c] + dt * rr[c])) / (R)3;
+        if (k == 1) v = s0[ci + c] + h * rr[c];
+        else if (k == 2) v = (R)0.75 * s0[ci + c] + (R)0.25 * (s1[ci + c] + h * rr[c]);
+        else v = (s0[ci + c] + (R)2 * (s1[ci + c] + h * rr[c])) / (R)3;
         out[ci + c] = v;
       }
     }
+}
+
+void FN(stage_pass)(int k, int M, int N, R dt, const R* res, const R* s0, const R* s1, R* out) {
+  stage_cells(k, M, N, dt, NULL, res, s0, s1, out);
 }
 
 /* wave speeds for CFL (ausm.js:617-630) and their max (ausm.js:555-580) */
@@ -597,34 +603,44 @@ void FN(eval_residual)(const R* par, int scheme, int limiter, int M, int N, cons
  * every `refresh` steps (the reference refreshes once per frame of
  * stepsPerFrame steps, main.js:789-791).  step_count carries the position in
  * that cadence across calls.  res (optional, NxMx4) receives the stage-3
- * residual of the last step.  Returns the dt of the last step. */
-R FN(run_steps)(const R* par, int scheme, int limiter, int M, int N, const R* vtx,
-                const R* flen, const R* area, R* s0, R* s1, R* s2, int nsteps, int refresh,
-                long long* step_count, R* maxws, R* res_out) {
+ * residual of the last step.  Returns the dt of the last step.
+ *
+ * local_dt != 0 (local time stepping, a build extension with no reference
+ * counterpart): every cell steps with its own dt = dt_from_ws(wave speed of
+ * the cell in Q^n), the same in all three stages; *maxws and the returned
+ * global dt are kept as in the global mode. */
+R FN(run_steps_dt)(const R* par, int scheme, int limiter, int M, int N, const R* vtx,
+                   const R* flen, const R* area, R* s0, R* s1, R* s2, int nsteps, int refresh,
+                   long long* step_count, R* maxws, R* res_out, int local_dt) {
   R* fx = (R*)malloc(sizeof(R) * 4 * (size_t)(M + 1) * N);
   R* fy = (R*)malloc(sizeof(R) * 4 * (size_t)M * (N + 1));
   R* res = (R*)malloc(sizeof(R) * 4 * (size_t)M * N);
+  R* dtc = local_dt ? (R*)malloc(sizeof(R) * (size_t)M * N) : NULL;
   R dt = 0;
   for (int s = 0; s < nsteps; ++s) {
     dt = FN(dt_from_ws)(par, *maxws);
+    if (dtc) {
+      (void)FN(max_wave_speed)(par, M, N, s2, flen, area, dtc);
+      for (size_t c = 0; c < (size_t)M * N; ++c) dtc[c] = FN(dt_from_ws)(par, dtc[c]);
+    }
     /* stage 1: state2 -> state0 (Qn), Q1 -> state1 */
     FN(boundary_pass)(par, M, N, vtx, s2, s0);
     FN(flux_pass)(par, scheme, limiter, 1, M, N, vtx, s0, fy);
     FN(flux_pass)(par, scheme, limiter, 0, M, N, vtx, s0, fx);
     FN(residual_pass)(M, N, fx, fy, flen, area, res);
-    FN(stage_pass)(1, M, N, dt, res, s0, NULL, s1);
+    stage_cells(1, M, N, dt, dtc, res, s0, NULL, s1);
     /* stage 2: state1 -> state2 (Q1), Q2 -> state2 */
     FN(boundary_pass)(par, M, N, vtx, s1, s2);
     FN(flux_pass)(par, scheme, limiter, 1, M, N, vtx, s2, fy);
     FN(flux_pass)(par, scheme, limiter, 0, M, N, vtx, s2, fx);
     FN(residual_pass)(M, N, fx, fy, flen, area, res);
-    FN(stage_pass)(2, M, N, dt, res, s0, s1, s2);
+    stage_cells(2, M, N, dt, dtc, res, s0, s1, s2);
     /* stage 3: state2 -> state1 (Q2), Qn+1 -> state2 */
     FN(boundary_pass)(par, M, N, vtx, s2, s1);
     FN(flux_pass)(par, scheme, limiter, 1, M, N, vtx, s1, fy);
     FN(flux_pass)(par, scheme, limiter, 0, M, N, vtx, s1, fx);
     FN(residual_pass)(M, N, fx, fy, flen, area, res);
-    FN(stage_pass)(3, M, N, dt, res, s0, s1, s2);
+    stage_cells(3, M, N, dt, dtc, res, s0, s1, s2);
     *step_count += 1;
     if (refresh > 0 && (*step_count % refresh) == 0)
       *maxws = FN(max_wave_speed)(par, M, N, s2, flen, area, NULL);
@@ -633,6 +649,113 @@ R FN(run_steps)(const R* par, int scheme, int limiter, int M, int N, const R* vt
   free(fx);
   free(fy);
   free(res);
+  free(dtc);
+  return dt;
+}
+
+R FN(run_steps)(const R* par, int scheme, int limiter, int M, int N, const R* vtx,
+                const R* flen, const R* area, R* s0, R* s1, R* s2, int nsteps, int refresh,
+                long long* step_count, R* maxws, R* res_out) {
+  return FN(run_steps_dt)(par, scheme, limiter, M, N, vtx, flen, area, s0, s1, s2, nsteps, refresh,
+                          step_count, maxws, res_out, 0);
+}
+
+/* Compensated-state TVD-RK3 (a build extension with no reference
+ * counterpart).  The same Shu-Osher scheme in increment form,
+ *   K_s = dt L(Q_{s-1}),  Q1 = Q^n + K1,  Q2 = Q^n + (K1 + K2)/4,
+ *   Q^{n+1} = Q^n + (K1 + K2 + 4 K3)/6,
+ * with the solution held as the unevaluated sum hi + lo of two R values:
+ * the final increment is added with an error-free TwoSum and the rounding
+ * error carried in lo, so that increments below half an ulp of hi are not
+ * rounded away (the stagnation of plain R accumulation near a steady
+ * state).  Fluxes see hi only (round(hi + lo) = hi).  s0/s1/s2 are the
+ * three state textures as in run_steps (s2 holds hi on entry and exit);
+ * lo is (N+3) x M x 4 (interior rows used) and acc N x M x 4 scratch.
+ * Per cell and component, in this order:
+ *   stage 1: K = dt*res;  acc = K;                 Q1 = hi + K
+ *   stage 2: K = dt*res;  acc = acc + K;           Q2 = hi + 0.25*acc
+ *   stage 3: K = dt*res;  D = (acc + 4*K) / 6;
+ *            s = hi + D;  b = s - hi;  e = (hi - (s - b)) + (D - b);
+ *            t = lo + e;  h = s + t;   b2 = h - s;
+ *            lo = (s - (h - b2)) + (t - b2);  hi = h. */
+static void comp_stage(int k, int M, int N, R dt, const R* dtc, const R* res, const R* hi, R* lo,
+                       R* acc, R* out) {
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < M; ++x) {
+      const size_t ci = ((size_t)(y + 1) * M + x) * 4;
+      const size_t ri = ((size_t)y * M + x) * 4;
+      const R h = dtc ? dtc[(size_t)y * M + x] : dt;
+      for (int c = 0; c < 4; ++c) {
+        const R K = h * res[ri + c];
+        if (k == 1) {
+          acc[ri + c] = K;
+          out[ci + c] = hi[ci + c] + K;
+        } else if (k == 2) {
+          const R a = acc[ri + c] + K;
+          acc[ri + c] = a;
+          out[ci + c] = hi[ci + c] + (R)0.25 * a;
+        } else {
+          const R D = (acc[ri + c] + (R)4 * K) / (R)6;
+          const R q = hi[ci + c];
+          const R sm = q + D;
+          const R b = sm - q;
+          const R e = (q - (sm - b)) + (D - b);
+          const R t = lo[ci + c] + e;
+          const R hn = sm + t;
+          const R b2 = hn - sm;
+          lo[ci + c] = (sm - (hn - b2)) + (t - b2);
+          out[ci + c] = hn;
+        }
+      }
+    }
+}
+
+R FN(run_steps_comp)(const R* par, int scheme, int limiter, int M, int N, const R* vtx,
+                     const R* flen, const R* area, R* s0, R* s1, R* s2, R* lo, int nsteps,
+                     int refresh, long long* step_count, R* maxws, int local_dt) {
+  R* fx = (R*)malloc(sizeof(R) * 4 * (size_t)(M + 1) * N);
+  R* fy = (R*)malloc(sizeof(R) * 4 * (size_t)M * (N + 1));
+  R* res = (R*)malloc(sizeof(R) * 4 * (size_t)M * N);
+  R* acc = (R*)malloc(sizeof(R) * 4 * (size_t)M * N);
+  R* hi = (R*)malloc(sizeof(R) * 4 * (size_t)M * (N + 3));
+  R* dtc = local_dt ? (R*)malloc(sizeof(R) * (size_t)M * N) : NULL;
+  R dt = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    dt = FN(dt_from_ws)(par, *maxws);
+    if (dtc) {
+      (void)FN(max_wave_speed)(par, M, N, s2, flen, area, dtc);
+      for (size_t c = 0; c < (size_t)M * N; ++c) dtc[c] = FN(dt_from_ws)(par, dtc[c]);
+    }
+    memcpy(hi, s2, sizeof(R) * 4 * (size_t)M * (N + 3));
+    /* stage 1: hi -> s0 (with ghosts), Q1 -> s1 */
+    FN(boundary_pass)(par, M, N, vtx, hi, s0);
+    FN(flux_pass)(par, scheme, limiter, 1, M, N, vtx, s0, fy);
+    FN(flux_pass)(par, scheme, limiter, 0, M, N, vtx, s0, fx);
+    FN(residual_pass)(M, N, fx, fy, flen, area, res);
+    comp_stage(1, M, N, dt, dtc, res, hi, lo, acc, s1);
+    /* stage 2: Q1 -> Q2 */
+    FN(boundary_pass)(par, M, N, vtx, s1, s2);
+    FN(flux_pass)(par, scheme, limiter, 1, M, N, vtx, s2, fy);
+    FN(flux_pass)(par, scheme, limiter, 0, M, N, vtx, s2, fx);
+    FN(residual_pass)(M, N, fx, fy, flen, area, res);
+    comp_stage(2, M, N, dt, dtc, res, hi, lo, acc, s2);
+    /* stage 3: Q2 -> (hi, lo) */
+    FN(boundary_pass)(par, M, N, vtx, s2, s1);
+    FN(flux_pass)(par, scheme, limiter, 1, M, N, vtx, s1, fy);
+    FN(flux_pass)(par, scheme, limiter, 0, M, N, vtx, s1, fx);
+    FN(residual_pass)(M, N, fx, fy, flen, area, res);
+    comp_stage(3, M, N, dt, dtc, res, hi, lo, acc, s2);
+    *step_count += 1;
+    if (refresh > 0 && (*step_count % refresh) == 0)
+      *maxws = FN(max_wave_speed)(par, M, N, s2, flen, area, NULL);
+  }
+  free(fx);
+  free(fy);
+  free(res);
+  free(acc);
+  free(hi);
+  free(dtc);
   return dt;
 }
 

@@ -44,6 +44,38 @@ void geo_naca4_point(double m, double p, double thick, double size, double x_off
   }
 }
 
+/* Kulfan class-shape transformation section (no reference counterpart;
+ * the reference draws NACA 4-digit bodies only).  Shape function evaluated
+ * from its definition: Bernstein polynomials of degree n = ncoef-1 with
+ * binomial coefficients from a factorial table, class function
+ * x^0.5 (1-x)^1; chord stations and placement as geo_naca4_point.  Pinned
+ * by the mathematics (partition of unity, degree elevation, nose radius,
+ * symmetry, trailing-edge gap) in tests/test_oracle_pins.py. */
+static double geo_cst_y(const double* a, int ncoef, double x) {
+  const int n = ncoef - 1;
+  double fact[33];
+  fact[0] = 1.0;
+  for (int k = 1; k <= n; ++k) fact[k] = fact[k - 1] * (double)k;
+  double s = 0.0;
+  for (int i = 0; i <= n; ++i)
+    s += a[i] * (fact[n] / (fact[i] * fact[n - i])) * pow(x, (double)i) * pow(1.0 - x, (double)(n - i));
+  return pow(x, 0.5) * pow(1.0 - x, 1.0) * s;
+}
+
+void geo_cst_point(const double* au, const double* al, int ncoef, double te_gap, double size,
+                   double x_offset, double expo, int spacing, int M, int i, double* xy) {
+  const double t = (double)i / ((double)M / 2.0);
+  const double chord = 2.0 * size;
+  const int upper = t <= 1.0;
+  double xc;
+  if (spacing == 0) xc = upper ? pow(1.0 - t, expo) : pow(t - 1.0, expo);
+  else xc = upper ? 0.5 * (1.0 + cos(M_PI * t)) : 0.5 * (1.0 - cos(M_PI * (t - 1.0)));
+  const double y = upper ? geo_cst_y(au, ncoef, xc) + 0.5 * te_gap * xc
+                         : geo_cst_y(al, ncoef, xc) - 0.5 * te_gap * xc;
+  xy[0] = (xc - 0.5) * chord + x_offset;
+  xy[1] = y * chord;
+}
+
 /* generateCircularOuterBoundary (src/grid.js:232-237) */
 void geo_circle_point(double radius, int M, int i, double* xy) {
   const double t = (double)i / (double)M;

@@ -67,11 +67,18 @@ def _declare(L):
         g("eval_residual").argtypes = [vp, ci, ci, ci, ci, vp, vp, vp, vp, vp]
         g("run_steps").argtypes = [vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, vp, vp, vp]
         g("run_steps").restype = cr
+        g("run_steps_dt").argtypes = [vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, vp, vp,
+                                      vp, ci]
+        g("run_steps_dt").restype = cr
+        g("run_steps_comp").argtypes = [vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci,
+                                        vp, vp, ci]
+        g("run_steps_comp").restype = cr
         g("residual_norm").argtypes = [ci, ci, vp, ci]
         g("residual_norm").restype = cd
     for n in ("geo_naca4_point",):
         getattr(L, n).argtypes = [cd, cd, cd, cd, cd, cd, ci, ci, ci, vp]
     L.geo_circle_point.argtypes = [cd, ci, ci, vp]
+    L.geo_cst_point.argtypes = [vp, vp, ci, cd, cd, cd, cd, ci, ci, ci, vp]
     L.geo_rect_point.argtypes = [cd, ci, ci, vp]
 
 
@@ -126,6 +133,17 @@ def naca4_boundary(M, m, p, t, size=0.5, x_offset=0.0, expo=1.4, spacing=0):
     L = lib()
     for i in range(M):
         L.geo_naca4_point(m, p, t, size, x_offset, expo, spacing, M, i, _ptr(out[i]))
+    return out
+
+
+def cst_boundary(M, a_upper, a_lower, te_gap=0.0, size=0.5, x_offset=0.0, expo=1.4, spacing=1):
+    au = np.ascontiguousarray(a_upper, dtype=np.float64)
+    al = np.ascontiguousarray(a_lower, dtype=np.float64)
+    assert au.shape == al.shape
+    out = np.zeros((M, 2), dtype=np.float64)
+    for i in range(M):
+        lib().geo_cst_point(_ptr(au), _ptr(al), au.size, te_gap, size, x_offset, expo, spacing, M,
+                            i, _ptr(out[i]))
     return out
 
 
@@ -253,8 +271,11 @@ class Solver:
     """The reference's three-texture step driver on the oracle."""
 
     def __init__(self, prec, par, scheme, limiter, vtx, refresh=1, freestream=True,
-                 vtx_metric=None):
+                 vtx_metric=None, local_dt=False, comp=False):
+        """comp: compensated-state stages (solution = state + lo)."""
         self.o = Oracle(prec)
+        self.local_dt = int(bool(local_dt))
+        self.comp = bool(comp)
         self.par = self.o.arr(par)
         self.scheme, self.limiter = scheme, limiter
         self.vtx = self.o.arr(vtx)
@@ -269,21 +290,31 @@ class Solver:
         self.maxws = np.zeros(1, dtype=self.o.dt)
         self.maxws[0] = self.o.max_ws(self.par, self.s2, self.area, self.flen)
         self.last_dt = None
+        self.lo = np.zeros_like(self.s2)
 
     def set_state(self, st):
         st = self.o.arr(st)
         self.s0[...] = st
         self.s1[...] = st
         self.s2[...] = st
+        self.lo[...] = 0
         self.maxws[0] = self.o.max_ws(self.par, self.s2, self.area, self.flen)
 
     def step(self, n, want_res=False):
+        if self.comp:
+            assert not want_res
+            self.last_dt = self.o.f("run_steps_comp")(
+                _ptr(self.par), self.scheme, self.limiter, self.M, self.N, _ptr(self.vtx),
+                _ptr(self.flen), _ptr(self.area), _ptr(self.s0), _ptr(self.s1), _ptr(self.s2),
+                _ptr(self.lo), n, self.refresh, ctypes.byref(self.count), _ptr(self.maxws),
+                self.local_dt)
+            return None
         res = np.zeros((self.N, self.M, 4), dtype=self.o.dt) if want_res else None
-        dt = self.o.f("run_steps")(_ptr(self.par), self.scheme, self.limiter, self.M, self.N,
-                                   _ptr(self.vtx), _ptr(self.flen), _ptr(self.area),
-                                   _ptr(self.s0), _ptr(self.s1), _ptr(self.s2), n, self.refresh,
-                                   ctypes.byref(self.count), _ptr(self.maxws),
-                                   _ptr(res) if want_res else None)
+        dt = self.o.f("run_steps_dt")(_ptr(self.par), self.scheme, self.limiter, self.M, self.N,
+                                      _ptr(self.vtx), _ptr(self.flen), _ptr(self.area),
+                                      _ptr(self.s0), _ptr(self.s1), _ptr(self.s2), n,
+                                      self.refresh, ctypes.byref(self.count), _ptr(self.maxws),
+                                      _ptr(res) if want_res else None, self.local_dt)
         self.last_dt = dt
         return res
 

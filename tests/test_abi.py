@@ -66,3 +66,17 @@ def test_far_field_generators_match_oracle():
 
     assert np.allclose(circle_boundary(100, 20.0), o_circle(100, 20.0), atol=1e-14)
     assert np.allclose(rect_boundary(512, 1.5), o_rect(512, 1.5), atol=1e-15)
+
+
+@pytest.mark.parametrize("spacing", ["power", "cosine"])
+def test_cst_generator_matches_oracle(spacing):
+    from ausm import cst_boundary
+
+    from oracle_lib import cst_boundary as o_cst
+
+    au = [0.13, 0.13, 0.16, 0.14, 0.18, 0.20]
+    al = [-0.13, -0.14, -0.10, -0.03, 0.03, 0.06]
+    a = cst_boundary(512, au, al, te_gap=0.002, size=0.5, x_offset=0.1, spacing=spacing)
+    b = o_cst(512, au, al, te_gap=0.002, size=0.5, x_offset=0.1,
+              spacing=1 if spacing == "cosine" else 0)
+    assert np.allclose(a, b, rtol=0, atol=1e-15)

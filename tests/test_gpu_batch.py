@@ -25,9 +25,10 @@ CASES = [
 ]
 
 
-def single_run(c, vtx, steps, math, tile=0):
+def single_run(c, vtx, steps, math, tile=0, dt_mode="global"):
     s = ausm.Solver(c["M"], c["N"])
     s.set_tile(tile)
+    s.set_dt_mode(dt_mode)
     s.set_vertices(vtx)
     s.set_flow(c["mach"], c["aoa"])
     s.set_scheme(c["scheme"], ausm.VANALBADA, 0.0, 0.75)
@@ -40,22 +41,24 @@ def single_run(c, vtx, steps, math, tile=0):
     return out
 
 
-@pytest.mark.parametrize("math,sched,tile", [("exact", (0, 0), 0), ("fast", (0, 0), 0),
-                                             ("fast", (4, 2), 1), ("fast", (1, 1), 2)])
-def test_batch_matches_single_grid_bitwise(math, sched, tile):
-    """A batch with any schedule equals single-grid runs with any tile shape."""
+@pytest.mark.parametrize("math,sched,tile,dt_mode", [
+    ("exact", (0, 0), 0, "global"), ("fast", (0, 0), 0, "global"), ("fast", (4, 2), 1, "global"),
+    ("fast", (1, 1), 2, "global"), ("exact", (0, 0), 0, "local"), ("fast", (2, 4), 4, "local")])
+def test_batch_matches_single_grid_bitwise(math, sched, tile, dt_mode):
+    """A batch with any schedule equals single-grid runs with any tile shape,
+    with the global and the local time step."""
     steps = 60
     grids = [grid(c["M"], c["N"], c["code"]) for c in CASES]
     with ausm.Batch(max_solves=8, math=math) as b:
         b.set_schedule(*sched)
         ids = [b.submit(g, ausm.freestream(c["mach"], c["aoa"]), scheme=c["scheme"],
-                        max_steps=steps) for c, g in zip(CASES, grids)]
+                        max_steps=steps, dt_mode=dt_mode) for c, g in zip(CASES, grids)]
         b.advance(steps, check_every=20)
         for c, g, sid in zip(CASES, grids, ids):
             st = b.poll(sid)
             assert st["state"] == "capped" and st["steps"] == steps
             got = b.state(sid)
-            ref = single_run(c, g, steps, math, tile)
+            ref = single_run(c, g, steps, math, tile, dt_mode)
             N = c["N"]
             assert np.array_equal(got[1:N + 1], ref[1:N + 1]), c
 

@@ -188,3 +188,19 @@ def test_precision_switch_carries_the_solution():
     buf = np.zeros((N + 3, M, 4), dtype=np.float64)
     assert s._L.ausm_get_state64(s.h, buf.ctypes.data_as(ctypes.c_void_p)) == -4
     s.close()
+
+
+def test_fp64_local_dt_bitwise():
+    M, N = 96, 40
+    vtx = grid64(M, N)
+    s, par = new_fp64(vtx, 0.73, 2.5, SLAU2, VANALBADA)
+    s.set_dt_mode("local")
+    ref = Solver(64, par, SLAU2, VANALBADA, vtx, local_dt=True)
+    st = perturbed_state64(par, M, N, seed=23)
+    s.set_state(st)
+    ref.set_state(st)
+    for chunk in (1, 40):
+        s.step(chunk)
+        ref.step(chunk)
+        assert np.array_equal(s.state()[1:N + 1], ref.state[1:N + 1]), chunk
+    s.close()

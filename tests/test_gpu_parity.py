@@ -354,3 +354,42 @@ def test_nonfinite_is_flagged():
         s.step(2)
     assert e.value.code == -5
     s.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_local_dt_exact_bitwise(scheme):
+    """Local time stepping (each cell its own CFL step from Q^n) against the
+    oracle's local mode, across a captured graph and direct launches."""
+    M, N = 96, 40
+    vtx, _, _ = make_grid(M, N, code=2412, iters=150)
+    s, _, par = new_pair(vtx, 0.73, 2.5, scheme, VANALBADA)
+    s.set_dt_mode("local")
+    ref = Solver(32, par, scheme, VANALBADA, vtx, local_dt=True)
+    st = perturbed_state(par, M, N, seed=31 + scheme)
+    s.set_state(st)
+    ref.set_state(st)
+    for chunk in (1, 7, 40):
+        info = s.step(chunk)
+        ref.step(chunk)
+        assert np.array_equal(s.state()[1:N + 1], ref.state[1:N + 1]), chunk
+        assert info["dt"] == ref.last_dt  # the global dt is still reported
+    s.close()
+
+
+def test_local_dt_fast_within_tolerance():
+    M, N = 128, 48
+    vtx, _, _ = make_grid(M, N, code=2412, iters=150)
+    s, _, par = new_pair(vtx, 0.8, 1.25, SLAU2, VANALBADA, math_mode="fast")
+    s.set_dt_mode("local")
+    ref = Solver(32, par, SLAU2, VANALBADA, vtx, local_dt=True)
+    st = perturbed_state(par, M, N, seed=4)
+    s.set_state(st)
+    ref.set_state(st)
+    n_ref = rho_norm(Oracle(32).residual(par, SLAU2, VANALBADA, vtx, st))
+    for k in range(30):
+        info = s.step(1)
+        ref.step(1)
+        assert abs(info["res_l2"] - n_ref) <= FAST_TOL_NORM * n_ref, (k, info["res_l2"], n_ref)
+        n_ref = rho_norm(Oracle(32).residual(par, SLAU2, VANALBADA, vtx, ref.state))
+    assert max_rel(s.state()[1:N + 1], ref.state[1:N + 1]) < 1e-4
+    s.close()

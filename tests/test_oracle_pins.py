@@ -16,8 +16,8 @@ import numpy as np
 import pytest
 
 from oracle_lib import (AUSMUP, MINMOD, NONE, SLAU, SLAU2, VANALBADA, VANLEER, P_GAMMA, P_INP,
-                        P_IS0, Oracle, Solver, circle_boundary, make_params, naca4_boundary,
-                        naca4_digits)
+                        P_IS0, Oracle, Solver, circle_boundary, cst_boundary, make_params,
+                        naca4_boundary, naca4_digits)
 
 SCHEMES = [SLAU2, SLAU, AUSMUP]
 LIMITERS = [VANALBADA, MINMOD, VANLEER, NONE]
@@ -346,6 +346,42 @@ def test_naca_textbook_ordinates():
     assert b2[M // 4, 1] > 0 > b2[3 * M // 4, 1]
 
 
+def test_cst_closed_forms():
+    """Kulfan CST: equal coefficients reduce the Bernstein sum to 1 (partition
+    of unity) so y = A sqrt(x)(1-x); degree-elevated coefficients give the
+    same surface; the nose radius is A_0^2/2; mirrored coefficients mirror
+    the section; the trailing-edge gap is te_gap."""
+    M = 1024
+    A = 0.17
+    b = cst_boundary(M, [A] * 6, [-A] * 6, size=0.5, expo=1.4, spacing=0)
+    x = b[:, 0] + 0.5
+    up = np.arange(M) <= M // 2
+    sgn = np.where(up, 1.0, -1.0)
+    assert np.allclose(b[:, 1], sgn * A * np.sqrt(x) * (1 - x), rtol=0, atol=2e-16)
+    # degree elevation n=5 -> 6: A'_i = i/6 A_{i-1} + (1 - i/6) A_i
+    au = np.array([0.13, 0.13, 0.16, 0.14, 0.18, 0.20])
+    al = np.array([-0.13, -0.14, -0.10, -0.03, 0.03, 0.06])
+
+    def elevate(a):
+        n1 = a.size
+        return np.array([(i / n1) * (a[i - 1] if i > 0 else 0.0) +
+                         (1 - i / n1) * (a[i] if i < n1 else 0.0) for i in range(n1 + 1)])
+
+    b5 = cst_boundary(M, au, al, spacing=1)
+    b6 = cst_boundary(M, elevate(au), elevate(al), spacing=1)
+    assert np.allclose(b5, b6, rtol=0, atol=1e-15)
+    # mirror: a_lower = -a_upper gives y_l(x) = -y_u(x) (indices i and M - i)
+    bm = cst_boundary(M, au, -au, spacing=1)
+    assert np.allclose(bm[1:M // 2, 1], -bm[M - 1:M // 2:-1, 1], rtol=0, atol=1e-16)
+    # nose radius: y^2 / (2x) -> A_0^2 / 2 at the first stations off the nose
+    bn = cst_boundary(4096, au, al, spacing=1)
+    xn, yn = bn[2047, 0] + 0.5, bn[2047, 1]
+    assert xn < 1e-5 and math.isclose(yn * yn / (2 * xn), au[0] ** 2 / 2, rel_tol=1e-4)
+    # trailing-edge gap
+    bt = cst_boundary(M, au, al, te_gap=0.004, spacing=1)
+    assert math.isclose(bt[0, 1], 0.002, abs_tol=1e-16) and bt[0, 0] == 0.5
+
+
 def test_ref32_tracks_ref64():
     """On a smooth subsonic case the FP32 and FP64 oracles agree to FP32
     rounding growth over a few dozen steps."""
@@ -388,3 +424,120 @@ def test_dt_lag_cadence():
     assert float(s.maxws[0]) != ws0
     assert float(s.maxws[0]) == o.max_ws(par, s.state, s.area, s.flen)
     assert par[P_GAMMA] == np.float32(1.4) and par[P_INP] == np.float32(1 / 1.4)
+
+
+def test_local_dt_reduces_to_global_when_maxdt_binds():
+    """Local time stepping with maxdt below every cell's CFL step takes dt =
+    maxdt everywhere, exactly the global step."""
+    o = Oracle(64)
+    v = _naca_grid(o, 48, 16, iters=60)
+    par = make_params(0.7, 2.0, maxdt=1e-5, dtype=np.float64)
+    area, flen = o.finalize(v)
+    assert float(par[9]) < float(par[10]) / o.max_ws(par, o.init_state(par, 48, 16), area, flen)
+    g = Solver(64, par, SLAU2, VANALBADA, v)
+    loc = Solver(64, par, SLAU2, VANALBADA, v, local_dt=True)
+    g.step(6)
+    loc.step(6)
+    assert np.array_equal(g.state, loc.state)
+
+
+def test_local_dt_is_the_cells_own_cfl_step():
+    """With a tiny CFL number one RK3 step is Q + dt_c L(Q) + O(dt^2): the
+    ratio of the density change to the density residual recovers each
+    cell's dt_c = cfl / ws_c (and the global mode's single min step)."""
+    o = Oracle(64)
+    M, N = 48, 16
+    v = _naca_grid(o, M, N, iters=60)
+    par = make_params(0.7, 2.0, cfl=1e-4, maxdt=1.0, dtype=np.float64)
+    st = o.init_state(par, M, N)
+    rng = np.random.default_rng(3)
+    st[..., 0] *= 1 + 0.01 * rng.standard_normal(st.shape[:2])
+    area, flen = o.finalize(v)
+    _, ws = o.max_ws(par, st, area, flen, want_field=True)
+    L0 = o.residual(par, SLAU2, VANALBADA, v, st)[..., 0]
+    mask = np.abs(L0) > 1e-2 * np.abs(L0).max()
+    for local, expect in ((True, 1e-4 / ws), (False, np.full_like(ws, 1e-4 / ws.max()))):
+        s = Solver(64, par, SLAU2, VANALBADA, v, local_dt=local)
+        s.set_state(st)
+        s.step(1)
+        ratio = (s.state[1:N + 1, :, 0] - st[1:N + 1, :, 0]) / L0
+        assert np.allclose(ratio[mask], expect[mask], rtol=1e-3, atol=0), local
+    assert ws.max() / ws.min() > 5  # the two modes differ on this grid
+
+
+def test_local_dt_shares_the_steady_state():
+    """Local time stepping converges to a fixed point of the discrete
+    operator, which the global-dt scheme then leaves unchanged."""
+    o = Oracle(64)
+    M, N = 40, 12
+    v = _naca_grid(o, M, N, iters=60)
+    par = make_params(0.5, 1.0, cfl=0.5, maxdt=1.0, dtype=np.float64)
+    loc = Solver(64, par, SLAU2, VANALBADA, v, local_dt=True)
+    r0 = np.sqrt((o.residual(par, SLAU2, VANALBADA, v, loc.state)[..., 0] ** 2).sum())
+    loc.step(6000)
+    r1 = np.sqrt((o.residual(par, SLAU2, VANALBADA, v, loc.state)[..., 0] ** 2).sum())
+    assert r1 < 1e-7 * r0, r1 / r0
+    g = Solver(64, par, SLAU2, VANALBADA, v)
+    g.set_state(loc.state)
+    g.step(200)
+    q0 = loc.state[1:N + 1]
+    assert np.abs(g.state[1:N + 1] - q0).max() < 1e-9 * np.abs(q0).max()
+
+
+def test_compensated_increment_form_is_the_same_scheme():
+    """In FP64 the compensated increment form Q^n + (K1 + K2 + 4 K3)/6 and the
+    reference's Shu-Osher stage form agree to round-off after many steps
+    (a wrong stage weight would not)."""
+    o = Oracle(64)
+    M, N = 48, 16
+    v = _naca_grid(o, M, N, iters=60)
+    par = make_params(0.7, 2.0, cfl=0.5, maxdt=1.0, dtype=np.float64)
+    st = o.init_state(par, M, N)
+    rng = np.random.default_rng(9)
+    st[..., 0] *= 1 + 0.02 * rng.standard_normal(st.shape[:2])
+    for local in (False, True):
+        a = Solver(64, par, SLAU2, VANALBADA, v, local_dt=local)
+        b = Solver(64, par, SLAU2, VANALBADA, v, local_dt=local, comp=True)
+        a.set_state(st)
+        b.set_state(st)
+        a.step(30)
+        b.step(30)
+        qa = a.state[1:N + 1]
+        qb = b.state[1:N + 1] + b.lo[1:N + 1]
+        assert np.abs(qa - qb).max() < 1e-12 * np.abs(qa).max(), local
+        assert abs(a.last_dt - b.last_dt) <= 1e-12 * a.last_dt
+
+
+def test_compensated_state_accumulates_sub_ulp_increments():
+    """With increments far below half an ulp of Q (cfl = 1e-12), plain FP32
+    accumulation cannot carry them: the state only moves by the round-off of
+    the stage combinations (at most an ulp, uncorrelated with L).  The
+    compensated pair accumulates them: after n steps hi is unchanged and lo
+    equals n dt L(Q0) (all stages see Q0, and (1 + 1 + 4)/6 = 1)."""
+    o32, o64 = Oracle(32), Oracle(64)
+    M, N = 48, 16
+    v = _naca_grid(o64, M, N, iters=60).astype(np.float32)
+    par = make_params(0.7, 2.0, cfl=1e-12, maxdt=1.0)
+    st = o32.init_state(par, M, N)
+    rng = np.random.default_rng(5)
+    st[..., 0] *= (1 + 0.02 * rng.standard_normal(st.shape[:2])).astype(np.float32)
+    n = 50
+    plain = Solver(32, par, SLAU2, VANALBADA, v, local_dt=True)
+    comp = Solver(32, par, SLAU2, VANALBADA, v, local_dt=True, comp=True)
+    plain.set_state(st)
+    comp.set_state(st)
+    plain.step(n)
+    comp.step(n)
+    q0 = st[1:N + 1].astype(np.float64)
+    q0f = st[1:N + 1]
+    ulp = np.spacing(np.abs(q0f)).astype(np.float64)
+    assert (np.abs(plain.state[1:N + 1].astype(np.float64) - q0) <= ulp).all()
+    assert np.array_equal(comp.state[1:N + 1], q0f)  # hi unchanged
+    moved = comp.lo[1:N + 1].astype(np.float64)
+    area, flen = o32.finalize(v)
+    _, ws = o32.max_ws(par, st, area, flen, want_field=True)
+    L = o32.residual(par, SLAU2, VANALBADA, v, st).astype(np.float64)
+    expect = n * (float(par[10]) / ws.astype(np.float64))[..., None] * L
+    mask = np.abs(expect) > 1e-3 * np.abs(expect).max()
+    assert np.allclose(moved[mask], expect[mask], rtol=1e-3, atol=0)
+    assert np.abs(moved).max() > 0
