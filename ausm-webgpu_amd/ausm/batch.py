@@ -16,13 +16,22 @@ STATES = {0: "queued", 1: "running", 2: "converged", 3: "capped", 4: "failed"}
 
 
 class Batch:
-    def __init__(self, max_solves=64, hbm_budget_bytes=0, device=0, math="fast"):
+    def __init__(self, max_solves=64, hbm_budget_bytes=0, device=0, math="fast",
+                 precision="fp32"):
         self._L = L.load()
         h = ctypes.c_void_p()
         check(self._L.ausm_batch_create(device, max_solves, hbm_budget_bytes, ctypes.byref(h)))
         self.h = h
         self.set_math(math)
+        self.set_precision(precision)
         self._keep = {}
+
+    def set_precision(self, prec):
+        """"fp32" or "fp32c" (compensated state) for every solve; before the
+        first submit."""
+        if isinstance(prec, str):
+            prec = {"fp32": L.PREC_FP32, "fp32c": L.PREC_FP32C}[prec]
+        check(self._L.ausm_batch_set_precision(self.h, prec))
 
     def close(self):
         if self.h:

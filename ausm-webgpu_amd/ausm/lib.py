@@ -103,6 +103,7 @@ SIGNATURES = {
     "ausm_batch_stream": (_i, [_vp, _pp]),
     "ausm_batch_set_math": (_i, [_vp, _i]),
     "ausm_batch_set_schedule": (_i, [_vp, _i, _i]),
+    "ausm_batch_set_precision": (_i, [_vp, _i]),
     "ausm_batch_submit": (_i, [_vp, ctypes.POINTER(SolveDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ausm_batch_advance": (_i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_int)]),
     "ausm_batch_poll": (_i, [_vp, ctypes.c_uint64, ctypes.POINTER(SolveStatus)]),

@@ -180,7 +180,8 @@ def solve(vtx, mach, aoa, prec="fp32", math_mode="exact", cfl=0.5, drop=1e-6,
 LIMITER_NAMES = {"vanalbada": L.VANALBADA, "minmod": L.MINMOD, "vanleer": L.VANLEER,
                  "none": L.NOLIMITER}
 DEFAULT_RUNS = tuple((p, m, lim) for lim in ("vanalbada", "none")
-                     for p, m in (("fp64", "exact"), ("fp32", "exact"), ("fp32", "fast")))
+                     for p, m in (("fp64", "exact"), ("fp32", "exact"), ("fp32", "fast"),
+                                  ("fp32c", "exact"), ("fp32c", "fast")))
 
 
 def run_study(M=1024, N=768, mach=0.73, aoa=2.5, seed=0, thickness=0.11, radius=20.0,

@@ -109,7 +109,7 @@ struct Slot {
 };
 
 struct BGraph {
-  int steps, ty, blocks, math;
+  int steps, ty, blocks, math, comp;
   hipGraph_t g;
   hipGraphExec_t e;
 };
@@ -117,12 +117,13 @@ struct BGraph {
 // sizes of one solve's arena block
 struct Layout {
   int P, PS, tiles_x, nws_max;
-  size_t q, v, ws, res, par, total;
+  size_t q, v, ws, res, par, comp, total;
 };
 
 static const int kBatchTiles[] = {1, 2, 4};
 
-static Layout layout_of(int M, int N) {
+// comp: two more state-sized plane sets (accumulator, low part)
+static Layout layout_of(int M, int N, bool comp) {
   Layout L{};
   L.P = (int)rup((size_t)M, 64);
   L.PS = (int)rup((size_t)N * L.P, 64);
@@ -135,7 +136,8 @@ static Layout layout_of(int M, int N) {
   L.ws = rup(sizeof(float) * (size_t)L.nws_max, 256);
   L.res = rup(sizeof(double) * (size_t)N * L.tiles_x, 256);
   L.par = rup(sizeof(DevParams) + sizeof(float), 256);
-  L.total = 3 * L.q + 2 * L.v + L.ws + L.res + L.par;
+  L.comp = comp ? 2 * L.q : 0;
+  L.total = 3 * L.q + 2 * L.v + L.ws + L.res + L.par + L.comp;
   return L;
 }
 
@@ -168,6 +170,7 @@ struct ausm_batch {
   int n_items = 0;
   bool dirty = false;
   int math = AUSM_MATH_FAST;
+  bool comp = false;  // AUSM_PREC_FP32C: compensated-state stages
   int fixed_ty = 0, fixed_bpc = 0;
   int ty = 2, bpc = 2;
   std::map<int, std::pair<int, int>> tuned;  // mix class -> (ty, bpc)
@@ -237,7 +240,7 @@ static int seed_ws(ausm_batch* b, Slot* s) {
     if (hipMalloc(&b->d_tmp, sizeof(float) * nb) != hipSuccess)
       return set_error(AUSM_ERR_NOMEM, "scratch allocation failed");
   }
-  const Layout L = layout_of(M, N);
+  const Layout L = layout_of(M, N, b->comp);
   if (launch_wave_speed(s->dev.q[0], s->dev.vx, s->dev.vy, s->dpar, M, N, L.P, L.PS, (float*)nullptr,
                         b->d_tmp, b->stream) != hipSuccess ||
       launch_batch_seed_ws(b->d_tmp, nb, s->dev.ws_part, L.nws_max, b->stream) != hipSuccess)
@@ -259,7 +262,7 @@ static int solve_norm(ausm_batch* b, Slot* s, double* out) {
   a.ws_out = s->dev.ws_part;
   a.res_out = s->dev.res_part;
   a.maxws_cur = s->dev.maxws_cur;
-  a.n_ws_in = layout_of(s->d.M, s->d.N).nws_max;
+  a.n_ws_in = layout_of(s->d.M, s->d.N, b->comp).nws_max;
   a.M = s->d.M;
   a.N = s->d.N;
   a.P = s->dev.P;
@@ -292,7 +295,7 @@ static int admit(ausm_batch* b) {
     if (idx < 0) return AUSM_OK;
     Slot* s = b->queue.front();
     const int M = s->d.M, N = s->d.N;
-    const Layout L = layout_of(M, N);
+    const Layout L = layout_of(M, N, b->comp);
     size_t off;
     if (!b->arena.alloc(L.total, off)) return AUSM_OK;  // wait for memory
     b->queue.pop_front();
@@ -311,6 +314,11 @@ static int admit(ausm_batch* b) {
     D.res_part = (double*)(p + 3 * L.q + 2 * L.v + L.ws);
     s->dpar = (DevParams*)(p + 3 * L.q + 2 * L.v + L.ws + L.res);
     D.maxws_cur = (float*)((char*)s->dpar + sizeof(DevParams));
+    if (b->comp) {
+      char* pc = p + 3 * L.q + 2 * L.v + L.ws + L.res + L.par;
+      D.acc = (float*)pc;
+      D.lo = (float*)(pc + L.q);
+    }
     D.M = M;
     D.N = N;
     D.P = L.P;
@@ -336,7 +344,8 @@ static int admit(ausm_batch* b) {
     if (hipMemcpyAsync(vx, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
         hipMemcpyAsync(vy, hy.data(), sizeof(float) * hy.size(), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
         hipMemcpyAsync(s->dpar, &D.par, sizeof(DevParams), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        launch_fill_state(D.q[0], s->dpar, M, N, L.P, L.PS, s->d.init_mode, b->stream) != hipSuccess)
+        launch_fill_state(D.q[0], s->dpar, M, N, L.P, L.PS, s->d.init_mode, b->stream) != hipSuccess ||
+        (b->comp && hipMemsetAsync(D.lo, 0, L.q, b->stream) != hipSuccess))
       return set_error(AUSM_ERR_HIP, "solve upload failed");
     int r = seed_ws(b, s);
     if (r != AUSM_OK) return r;
@@ -359,11 +368,11 @@ static float time_schedule(ausm_batch* b, int ty, int bpc) {
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   const int blocks = b->cus * bpc;
-  for (int w = 0; w < 2; ++w) (void)launch_batch_stage(a, b->math, ty, blocks, b->stream);
+  for (int w = 0; w < 2; ++w) (void)launch_batch_stage(a, b->math, b->comp, ty, blocks, b->stream);
   float ms = 1e30f;
   for (int round = 0; round < 2; ++round) {
     (void)hipEventRecord(e0, b->stream);
-    for (int k = 0; k < 3; ++k) (void)launch_batch_stage(a, b->math, ty, blocks, b->stream);
+    for (int k = 0; k < 3; ++k) (void)launch_batch_stage(a, b->math, b->comp, ty, blocks, b->stream);
     (void)hipEventRecord(e1, b->stream);
     (void)hipEventSynchronize(e1);
     float t = 1e30f;
@@ -429,7 +438,7 @@ static hipError_t enqueue_steps(ausm_batch* b, int nsteps) {
     for (int st = 1; st <= 3 && e == hipSuccess; ++st) {
       // the convergence check after the chunk reads the last step's norm
       BatchArgs a{b->d_solves, b->d_items, b->d_counts, st, st == 3, st == 1 && s == nsteps - 1};
-      e = launch_batch_stage(a, b->math, b->ty, blocks, b->stream);
+      e = launch_batch_stage(a, b->math, b->comp, b->ty, blocks, b->stream);
     }
     if (e != hipSuccess) return e;
   }
@@ -440,9 +449,11 @@ static int run_chunk(ausm_batch* b, int nsteps) {
   const int blocks = b->cus * b->bpc;
   BGraph* g = nullptr;
   for (auto& x : b->graphs)
-    if (x.steps == nsteps && x.ty == b->ty && x.blocks == blocks && x.math == b->math) g = &x;
+    if (x.steps == nsteps && x.ty == b->ty && x.blocks == blocks && x.math == b->math &&
+        x.comp == (int)b->comp)
+      g = &x;
   if (!g) {
-    BGraph n{nsteps, b->ty, blocks, b->math, nullptr, nullptr};
+    BGraph n{nsteps, b->ty, blocks, b->math, (int)b->comp, nullptr, nullptr};
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return set_error(AUSM_ERR_HIP, "graph capture failed");
     const hipError_t e = enqueue_steps(b, nsteps);
@@ -575,6 +586,19 @@ int ausm_batch_set_math(ausm_batch* b, int math) {
   return AUSM_OK;
 }
 
+int ausm_batch_set_precision(ausm_batch* b, int precision) {
+  if (!b) return set_error(AUSM_ERR_ARG, "NULL batch");
+  if (precision != AUSM_PREC_FP32 && precision != AUSM_PREC_FP32C)
+    return set_error(AUSM_ERR_ARG, "batches run FP32 or FP32C precision");
+  const bool comp = precision == AUSM_PREC_FP32C;
+  if (comp == b->comp) return AUSM_OK;
+  if (!b->queue.empty() || n_running(b) > 0)
+    return set_error(AUSM_ERR_ORDER, "set the batch precision before submitting solves");
+  b->comp = comp;
+  b->tuned.clear();
+  return AUSM_OK;
+}
+
 int ausm_batch_set_schedule(ausm_batch* b, int rows_per_wave, int blocks_per_cu) {
   if (!b) return set_error(AUSM_ERR_ARG, "NULL batch");
   if (rows_per_wave != 0 && rows_per_wave != 1 && rows_per_wave != 2 && rows_per_wave != 4)
@@ -598,7 +622,7 @@ int ausm_batch_submit(ausm_batch* b, const ausm_solve_desc* d, uint64_t* id) {
     return set_error(AUSM_ERR_ARG, "bad init mode");
   if (d->dt_mode != AUSM_DT_GLOBAL && d->dt_mode != AUSM_DT_LOCAL)
     return set_error(AUSM_ERR_ARG, "bad dt mode");
-  const Layout L = layout_of(d->M, d->N);
+  const Layout L = layout_of(d->M, d->N, b->comp);
   if (L.total > b->arena.size())
     return set_error(AUSM_ERR_FULL, "a %d x %d solve needs %zu bytes, more than the whole budget", d->M, d->N, L.total);
   Slot* s = new Slot();

@@ -38,6 +38,8 @@ __device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, in
   a.limiter = S.limiter;
   a.tiles_x = S.tiles_x;
   a.local_dt = S.local_dt;
+  a.acc = S.acc;
+  a.lo = S.lo;
   a.fresh = 0;
   a.refresh = refresh;
   if (stage == 1) {
@@ -66,7 +68,7 @@ __device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, in
   return a;
 }
 
-template <bool FAST, int TY>
+template <bool FAST, int TY, bool COMP>
 __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
   const int n = *b.n_items;
   const int g = xcd_remap(blockIdx.x, gridDim.x);
@@ -83,9 +85,9 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
     const float dt = fminf(P.maxdt, P.cfl / *S->maxws_cur);
     float ws;
     switch (S->scheme) {
-      case SCHEME_SLAU2: ws = stage_tile<SCHEME_SLAU2, FAST, TY>(P, a, tx, tyb, dt); break;
-      case SCHEME_SLAU: ws = stage_tile<SCHEME_SLAU, FAST, TY>(P, a, tx, tyb, dt); break;
-      default: ws = stage_tile<SCHEME_AUSMUP, FAST, TY>(P, a, tx, tyb, dt); break;
+      case SCHEME_SLAU2: ws = stage_tile<SCHEME_SLAU2, FAST, TY, float, COMP>(P, a, tx, tyb, dt); break;
+      case SCHEME_SLAU: ws = stage_tile<SCHEME_SLAU, FAST, TY, float, COMP>(P, a, tx, tyb, dt); break;
+      default: ws = stage_tile<SCHEME_AUSMUP, FAST, TY, float, COMP>(P, a, tx, tyb, dt); break;
     }
     if (a.stage == 3 && a.refresh) block_ws_partial(ws, S->ws_part + item.tile);
   }
@@ -145,19 +147,21 @@ __global__ void batch_seed_ws_kernel(const float* part, int n_part, float* ws_pa
   for (int i = threadIdx.x; i < n_ws; i += 64) ws_part[i] = (i == 0) ? m : 0.0f;
 }
 
-template <bool FAST>
+template <bool FAST, bool COMP>
 static hipError_t launch_bt(const BatchArgs& b, int ty, int blocks, hipStream_t s) {
   switch (ty) {
-    case 1: hipLaunchKernelGGL((batch_stage_kernel<FAST, 1>), dim3(blocks), dim3(256), 0, s, b); break;
-    case 2: hipLaunchKernelGGL((batch_stage_kernel<FAST, 2>), dim3(blocks), dim3(256), 0, s, b); break;
-    case 4: hipLaunchKernelGGL((batch_stage_kernel<FAST, 4>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 1: hipLaunchKernelGGL((batch_stage_kernel<FAST, 1, COMP>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 2: hipLaunchKernelGGL((batch_stage_kernel<FAST, 2, COMP>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 4: hipLaunchKernelGGL((batch_stage_kernel<FAST, 4, COMP>), dim3(blocks), dim3(256), 0, s, b); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_batch_stage(const BatchArgs& b, int fast, int ty, int blocks, hipStream_t s) {
-  return fast ? launch_bt<true>(b, ty, blocks, s) : launch_bt<false>(b, ty, blocks, s);
+hipError_t launch_batch_stage(const BatchArgs& b, int fast, int comp, int ty, int blocks,
+                              hipStream_t s) {
+  if (comp) return fast ? launch_bt<true, true>(b, ty, blocks, s) : launch_bt<false, true>(b, ty, blocks, s);
+  return fast ? launch_bt<true, false>(b, ty, blocks, s) : launch_bt<false, false>(b, ty, blocks, s);
 }
 
 hipError_t launch_batch_dt(const SolveDev* solves, const int* n_solves, int max_solves,

@@ -16,6 +16,8 @@ struct SolveDev {
   float* ws_part;    // per-workgroup-tile max wave speeds (stage-3 layout)
   double* res_part;  // density-residual sums of squares per (row, strip)
   float* maxws_cur;  // max wave speed of this step's dt
+  float* acc;        // compensated batches: increment accumulator planes
+  float* lo;         // compensated batches: low part of the solution
   int M, N, P, PS, tiles_x;
   int scheme, limiter;
   int n_ws;    // workgroup tiles of this solve = entries of ws_part
@@ -37,7 +39,9 @@ struct BatchArgs {
   int norm;            // stage 1: write the residual-norm partials
 };
 
-hipError_t launch_batch_stage(const BatchArgs& b, int fast, int ty, int blocks, hipStream_t s);
+// comp: compensated-state stages (every solve of the batch)
+hipError_t launch_batch_stage(const BatchArgs& b, int fast, int comp, int ty, int blocks,
+                              hipStream_t s);
 hipError_t launch_batch_dt(const SolveDev* solves, const int* n_solves, int max_solves,
                            hipStream_t s);
 hipError_t launch_batch_norm(const SolveDev* solves, const int* n_solves, int max_solves,

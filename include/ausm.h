@@ -235,6 +235,10 @@ int ausm_batch_create(int device, int max_solves, size_t hbm_budget_bytes, ausm_
 int ausm_batch_destroy(ausm_batch* b);
 int ausm_batch_stream(ausm_batch* b, void** stream);
 int ausm_batch_set_math(ausm_batch* b, int math);
+/* AUSM_PREC_FP32 (default) or AUSM_PREC_FP32C (compensated state, see
+ * ausm_set_precision) for every solve of the batch; only before the first
+ * submit (or once the batch is empty) */
+int ausm_batch_set_precision(ausm_batch* b, int precision);
 /* fix the persistent schedule: rows per wavefront tile (1, 2, 4) and
  * workgroups per CU (1..16); 0 leaves a value to the per-mix autotuner */
 int ausm_batch_set_schedule(ausm_batch* b, int rows_per_wave, int blocks_per_cu);

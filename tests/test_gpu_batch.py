@@ -25,10 +25,11 @@ CASES = [
 ]
 
 
-def single_run(c, vtx, steps, math, tile=0, dt_mode="global"):
+def single_run(c, vtx, steps, math, tile=0, dt_mode="global", precision="fp32"):
     s = ausm.Solver(c["M"], c["N"])
     s.set_tile(tile)
     s.set_dt_mode(dt_mode)
+    s.set_precision(precision)
     s.set_vertices(vtx)
     s.set_flow(c["mach"], c["aoa"])
     s.set_scheme(c["scheme"], ausm.VANALBADA, 0.0, 0.75)
@@ -39,6 +40,29 @@ def single_run(c, vtx, steps, math, tile=0, dt_mode="global"):
     out = s.state()
     s.close()
     return out
+
+
+@pytest.mark.parametrize("math,sched,tile", [("exact", (0, 0), 0), ("fast", (2, 2), 4),
+                                             ("fast", (1, 4), 1)])
+def test_batch_compensated_matches_single_grid_bitwise(math, sched, tile):
+    """Compensated-state batches (local time step) equal single-grid FP32C
+    runs; hi is compared (the batch returns hi as the state)."""
+    steps = 60
+    grids = [grid(c["M"], c["N"], c["code"]) for c in CASES]
+    with ausm.Batch(max_solves=8, math=math, precision="fp32c") as b:
+        b.set_schedule(*sched)
+        ids = [b.submit(g, ausm.freestream(c["mach"], c["aoa"]), scheme=c["scheme"],
+                        max_steps=steps, dt_mode="local") for c, g in zip(CASES, grids)]
+        b.advance(steps, check_every=20)
+        for c, g, sid in zip(CASES, grids, ids):
+            got = b.state(sid)
+            ref = single_run(c, g, steps, math, tile, "local", "fp32c")
+            N = c["N"]
+            assert np.array_equal(got[1:N + 1], ref[1:N + 1]), c
+        # the precision is fixed while a solve is queued or running
+        b.submit(grids[0], ausm.freestream(0.5, 0.0), max_steps=10)
+        with pytest.raises(ausm.AusmError):
+            b.set_precision("fp32")
 
 
 @pytest.mark.parametrize("math,sched,tile,dt_mode", [
