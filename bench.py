@@ -187,9 +187,11 @@ def batched_leg(serve, n, steps, grid_iters):
     reqs = serve.make_requests(n, seed=2026, sizes=sizes, max_steps=steps, res_drop=0.0)
     serve.make_grids(reqs, iters=grid_iters)
     log(f"[bench] batched: {n} solves x {steps} steps")
-    bat = serve.run_batched(reqs, None, check_every=250, max_solves=n)
+    # the reference's stepping: global CFL step, plain FP32 state
+    bat = serve.run_batched(reqs, None, check_every=250, max_solves=n, dt_mode="global",
+                            precision="fp32")
     log(f"[bench] batched {bat['solves_per_s']:.2f} solves/s, {bat['cell_steps_per_s']:.3e} cell-steps/s")
-    seq = serve.run_sequential(reqs, None, check_every=250)
+    seq = serve.run_sequential(reqs, None, check_every=250, dt_mode="global", precision="fp32")
     log(f"[bench] one by one {seq['solves_per_s']:.2f} solves/s, {seq['cell_steps_per_s']:.3e} cell-steps/s")
     same = all(bat["results"][r["id"]]["state"] == seq["results"][r["id"]]["state"] and
                (bat["results"][r["id"]]["state"] == "failed" or
@@ -204,7 +206,16 @@ def batched_leg(serve, n, steps, grid_iters):
             "results_identical": same}
 
 
-def serving_leg(serve, n, cap, drop, rates, grid_iters):
+SERVE_VARIANTS = (("MUSCL van Albada (the reference's reconstruction)", 0),
+                  ("first order (MUSCL off)", 3))
+
+
+def serving_leg(serve, n, cap, drop, rates, grid_iters, dt_mode, precision, n_check):
+    """The same request stream served twice: with the reference's MUSCL van
+    Albada reconstruction (which settles into limit cycles at shocks and
+    extrema, so requests end at the step cap) and first order (which
+    converges): converged solves/s, p50/p99 latency, batched vs one by one,
+    and the error of converged results against the FP64 solve."""
     sizes = [(128, 64), (256, 128), (384, 192), (512, 384), (768, 512), (1024, 768)]
     reqs = serve.make_requests(n, seed=5, sizes=sizes, mach_range=(0.1, 10.0), log_mach=True,
                                aoa_range=(-5.0, 10.0), max_steps=cap, res_drop=drop)
@@ -212,28 +223,50 @@ def serving_leg(serve, n, cap, drop, rates, grid_iters):
     strip = lambda d: {k: v for k, v in d.items() if k != "results"}
     out = {"workload": f"{n} requests, sizes from {sizes}, Mach~logU[0.1,10], AoA~U[-5,10], "
                        f"residual drop {drop:g} or {cap}-step cap, checks every 100 steps",
-           "rates": []}
-    seq = serve.run_sequential(reqs, None, check_every=100)
-    out["one_by_one_closed_loop"] = strip(seq)
-    log(f"[bench] serving one by one: {seq['solves_per_s']:.2f} solves/s "
-        f"({seq['converged']} converged)")
-    for rate in rates:
-        arr = None if rate == 0 else serve.poisson_arrivals(n, rate, seed=11)
-        r = serve.run_batched(reqs, arr, check_every=100, max_solves=64)
-        rec = dict(strip(r), arrival_rate_per_s=("all at t=0" if rate == 0 else rate))
-        if rate:
-            sq = serve.run_sequential(reqs, arr, check_every=100)
-            rec["one_by_one_p50_s"] = sq["p50_latency_s"]
-            rec["one_by_one_p99_s"] = sq["p99_latency_s"]
-        out["rates"].append(rec)
-        log(f"[bench] serving rate {rate}: {r['solves_per_s']:.2f} solves/s, "
-            f"{r['converged_solves_per_s']:.2f} converged/s, p50 {r['p50_latency_s']:.3f} s, "
-            f"p99 {r['p99_latency_s']:.3f} s")
-    sat = out["rates"][0]
-    out["speedup_converged_solves_per_s"] = (
-        sat["converged_solves_per_s"] / seq["converged_solves_per_s"]
-        if seq["converged_solves_per_s"] > 0 else None)
-    out["speedup_solves_per_s"] = sat["solves_per_s"] / seq["solves_per_s"]
+           "dt_mode": dt_mode, "precision": precision, "variants": []}
+    for label, limiter in SERVE_VARIANTS:
+        for r in reqs:
+            r["limiter"] = limiter
+        var = {"reconstruction": label, "rates": []}
+        seq = serve.run_sequential(reqs, None, check_every=100, dt_mode=dt_mode,
+                                   precision=precision)
+        var["one_by_one_closed_loop"] = strip(seq)
+        log(f"[bench] serving ({label}) one by one: {seq['solves_per_s']:.2f} solves/s, "
+            f"{seq['converged']} converged")
+        sat = None
+        for rate in rates:
+            arr = None if rate == 0 else serve.poisson_arrivals(n, rate, seed=11)
+            r = serve.run_batched(reqs, arr, check_every=100, max_solves=64, dt_mode=dt_mode,
+                                  precision=precision)
+            rec = dict(strip(r), arrival_rate_per_s=("all at t=0" if rate == 0 else rate))
+            if rate:
+                sq = serve.run_sequential(reqs, arr, check_every=100, dt_mode=dt_mode,
+                                          precision=precision)
+                rec["one_by_one_p50_s"] = sq["p50_latency_s"]
+                rec["one_by_one_p99_s"] = sq["p99_latency_s"]
+            else:
+                sat = r
+            var["rates"].append(rec)
+            log(f"[bench] serving ({label}) rate {rate}: {r['solves_per_s']:.2f} solves/s, "
+                f"{r['converged_solves_per_s']:.2f} converged/s, p50 {r['p50_latency_s']:.3f} s, "
+                f"p99 {r['p99_latency_s']:.3f} s")
+        if sat is not None:
+            var["speedup_solves_per_s"] = sat["solves_per_s"] / seq["solves_per_s"]
+            var["speedup_converged_solves_per_s"] = (
+                sat["converged_solves_per_s"] / seq["converged_solves_per_s"]
+                if seq["converged_solves_per_s"] > 0 else None)
+            # served results equal the one-by-one results (same kernels, same
+            # check points)
+            a, b = sat["results"], seq["results"]
+            var["results_identical_to_one_by_one"] = all(
+                (a[k]["state"], a[k]["steps"], a[k].get("cl")) ==
+                (b[k]["state"], b[k]["steps"], b[k].get("cl")) for k in a)
+        if n_check > 0 and seq["converged"] > 0:
+            acc = serve.accuracy_vs_fp64(reqs, seq["results"], n_check=n_check, dt_mode=dt_mode)
+            var["accuracy_vs_fp64"] = acc
+            log(f"[bench] serving ({label}) accuracy vs FP64: max |dCl| {acc['max_abs_cl_err']}, "
+                f"max |dCd| {acc['max_abs_cd_err']} over {len(acc['checked'])} converged requests")
+        out["variants"].append(var)
     return out
 
 
@@ -253,6 +286,10 @@ def main():
     ap.add_argument("--serve-cap", type=int, default=20000)
     ap.add_argument("--serve-drop", type=float, default=1e-5)
     ap.add_argument("--serve-rates", default="0,4")
+    ap.add_argument("--serve-dt", default="local", choices=["local", "global"])
+    ap.add_argument("--serve-precision", default="fp32c", choices=["fp32c", "fp32"])
+    ap.add_argument("--serve-check", type=int, default=3,
+                    help="converged requests re-solved in FP64 for the accuracy figure")
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection run: three 512 MiB device copies (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE) then --steps steps of the workload, "
@@ -314,7 +351,7 @@ def main():
     if not args.no_serving:
         rates = [float(x) for x in args.serve_rates.split(",") if x != ""]
         serving = serving_leg(serve, args.serve_requests, args.serve_cap, args.serve_drop, rates,
-                              2000)
+                              2000, args.serve_dt, args.serve_precision, args.serve_check)
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")

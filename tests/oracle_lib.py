@@ -24,13 +24,28 @@ P_IS0, P_MAXDT, P_CFL, P_KP, P_KU = 5, 9, 10, 11, 12
 P_COUNT = 13
 
 
+LIBBASE = os.path.join(ORACLE_DIR, "libbaseline.so")
+
+
 def ensure_built():
-    if not os.path.exists(LIBORC):
+    if not os.path.exists(LIBORC) or not os.path.exists(LIBBASE):
         subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
     return LIBORC
 
 
 _lib = None
+_libomp = None
+
+
+def libomp():
+    """ref32 built -O3 with OpenMP over rows (oracle/libbaseline.so, prefix
+    base32_): the same FP32 code without contraction, for long runs."""
+    global _libomp
+    if _libomp is None:
+        ensure_built()
+        _libomp = ctypes.CDLL(LIBBASE)
+        _declare(_libomp, (("base32_", ctypes.c_float),), geometry=False)
+    return _libomp
 
 
 def lib():
@@ -45,9 +60,10 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def _declare(L):
+def _declare(L, prefixes=(("orc32_", ctypes.c_float), ("orc64_", ctypes.c_double)),
+             geometry=True):
     vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
-    for pfx, cr in (("orc32_", ctypes.c_float), ("orc64_", ctypes.c_double)):
+    for pfx, cr in prefixes:
         g = lambda n: getattr(L, pfx + n)
         g("face_flux").argtypes = [vp, ci, ci, vp, vp, vp, vp, cr, cr, vp]
         g("flux_pass").argtypes = [vp, ci, ci, ci, ci, ci, vp, vp, vp]
@@ -75,6 +91,8 @@ def _declare(L):
         g("run_steps_comp").restype = cr
         g("residual_norm").argtypes = [ci, ci, vp, ci]
         g("residual_norm").restype = cd
+    if not geometry:
+        return
     for n in ("geo_naca4_point",):
         getattr(L, n).argtypes = [cd, cd, cd, cd, cd, cd, ci, ci, ci, vp]
     L.geo_circle_point.argtypes = [cd, ci, ci, vp]
@@ -168,11 +186,17 @@ def rect_boundary(M, ratio):
 class Oracle:
     """One precision of the oracle (32 -> ref32, 64 -> ref64)."""
 
-    def __init__(self, prec=32):
+    def __init__(self, prec=32, omp=False):
+        """omp: ref32 from the OpenMP build (bitwise the same results)."""
         self.prec = prec
         self.dt = np.float32 if prec == 32 else np.float64
-        self.L = lib()
-        self.pfx = "orc32_" if prec == 32 else "orc64_"
+        if omp:
+            assert prec == 32
+            self.L = libomp()
+            self.pfx = "base32_"
+        else:
+            self.L = lib()
+            self.pfx = "orc32_" if prec == 32 else "orc64_"
 
     def f(self, name):
         return getattr(self.L, self.pfx + name)
@@ -271,9 +295,10 @@ class Solver:
     """The reference's three-texture step driver on the oracle."""
 
     def __init__(self, prec, par, scheme, limiter, vtx, refresh=1, freestream=True,
-                 vtx_metric=None, local_dt=False, comp=False):
-        """comp: compensated-state stages (solution = state + lo)."""
-        self.o = Oracle(prec)
+                 vtx_metric=None, local_dt=False, comp=False, omp=False):
+        """comp: compensated-state stages (solution = state + lo); omp: the
+        OpenMP build of ref32."""
+        self.o = Oracle(prec, omp=omp)
         self.local_dt = int(bool(local_dt))
         self.comp = bool(comp)
         self.par = self.o.arr(par)

@@ -541,3 +541,18 @@ def test_compensated_state_accumulates_sub_ulp_increments():
     mask = np.abs(expect) > 1e-3 * np.abs(expect).max()
     assert np.allclose(moved[mask], expect[mask], rtol=1e-3, atol=0)
     assert np.abs(moved).max() > 0
+
+
+def test_openmp_build_is_bitwise_ref32():
+    """The OpenMP -O3 build of ref32 (the CPU-baseline leg and the long-run
+    checker) gives bit-for-bit the results of the plain build."""
+    o = Oracle(64)
+    M, N = 64, 24
+    v = _naca_grid(o, M, N, iters=60).astype(np.float32)
+    par = make_params(0.8, 1.25, cfl=0.5, maxdt=1.0)
+    for local in (False, True):
+        a = Solver(32, par, SLAU2, MINMOD, v, local_dt=local)
+        b = Solver(32, par, SLAU2, MINMOD, v, local_dt=local, omp=True)
+        a.step(25)
+        b.step(25)
+        assert np.array_equal(a.state, b.state) and a.last_dt == b.last_dt
