@@ -274,12 +274,17 @@ __device__ __forceinline__ void muscl_fast(int limiter, float wl2, float wl, flo
     L = fmaf(phiL, hF, wl);
     Rr = fmaf(-phiR, hF, wr);
   } else {
-    // van Albada phi(r) = (r^2+r)/(r^2+1) with r = n/d:  n(n+d)/(n^2+d^2)
+    // van Albada phi(r) = (r^2+r)/(r^2+1) with r = n/d:  n(n+d)/(n^2+d^2);
+    // one reciprocal serves both sides: 1/den_L = den_R / (den_L den_R).
+    // den >= d^2 >= eps^2 keeps the product normal; if it overflows the
+    // reciprocal is 0 and both sides fall back to first order.
     const float d = fmaf(dF, dF, eps);
     const float d2 = d * d;
     const float nL = fmaf(dL, dF, eps), nR = fmaf(dF, dR, eps);
-    const float phiL = nL * (nL + d) * __builtin_amdgcn_rcpf(fmaf(nL, nL, d2));
-    const float phiR = nR * (nR + d) * __builtin_amdgcn_rcpf(fmaf(nR, nR, d2));
+    const float denL = fmaf(nL, nL, d2), denR = fmaf(nR, nR, d2);
+    const float r = __builtin_amdgcn_rcpf(denL * denR);
+    const float phiL = (nL * (nL + d)) * (denR * r);
+    const float phiR = (nR * (nR + d)) * (denL * r);
     L = fmaf(phiL, hF, wl);
     Rr = fmaf(-phiR, hF, wr);
   }
@@ -300,10 +305,14 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
   const float u_R = fmaf(R.u, nx, R.v * ny);
   const float vvL = fmaf(L.u, L.u, L.v * L.v);
   const float vvR = fmaf(R.u, R.u, R.v * R.v);
-  const float rcL = fmaxf(L.r, 1e-10f), rcR = fmaxf(R.r, 1e-10f);
-  const float iclL = __builtin_amdgcn_rcpf(rcL), iclR = __builtin_amdgcn_rcpf(rcR);
-  const float irL = L.r > 1e-10f ? iclL : __builtin_amdgcn_rcpf(L.r);
-  const float irR = R.r > 1e-10f ? iclR : __builtin_amdgcn_rcpf(R.r);
+  // 1/rho_L, 1/rho_R and 1/(rho_L + rho_R) from one reciprocal
+  const float rsum = L.r + R.r;
+  const float rprod = L.r * R.r;
+  const float rall = __builtin_amdgcn_rcpf(rprod * rsum);
+  const float irS = rprod * rall;
+  const float irL = (R.r * rsum) * rall, irR = (L.r * rsum) * rall;
+  // 1 / max(rho, 1e-10), the sound-speed guard
+  const float iclL = L.r > 1e-10f ? irL : 1e10f, iclR = R.r > 1e-10f ? irR : 1e10f;
   // h = p / ((gamma-1) rho) + |v|^2 / 2 + p / rho = gamma/(gamma-1) p/rho + |v|^2 / 2
   const float h_L = fmaf(P.ggm1, L.p * irL, 0.5f * vvL);
   const float h_R = fmaf(P.ggm1, R.p * irR, 0.5f * vvR);
@@ -313,10 +322,12 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float h_t = 0.5f * (h_L + h_R);
     const float aStar = __builtin_amdgcn_sqrtf(fmaxf(1e-6f, P.astar_coef * h_t));
     const float as2 = aStar * aStar;
-    const float aHat_L = as2 * __builtin_amdgcn_rcpf(fmaxf(u_L, aStar));
-    const float aHat_R = as2 * __builtin_amdgcn_rcpf(fmaxf(-u_R, aStar));
-    const float a_i = fminf(aHat_L, aHat_R);
-    const float ia = __builtin_amdgcn_rcpf(a_i);
+    // min(a*^2 / max(u_L, a*), a*^2 / max(-u_R, a*)) = a*^2 / max(u_L, -u_R, a*);
+    // a_i and 1/a_i from one reciprocal
+    const float mx = fmaxf(fmaxf(u_L, -u_R), aStar);
+    const float rq = __builtin_amdgcn_rcpf(mx * as2);
+    const float a_i = (as2 * as2) * rq;
+    const float ia = (mx * mx) * rq;
     const float M_L = u_L * ia;
     const float M_R = u_R * ia;
     const float a_i2 = a_i * a_i;
@@ -366,8 +377,7 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
       P_i = 0.5f * fmaf(fpp - fpm, L.p - R.p, fmaf(1.0f - X, fsum, 1.0f) * (L.p + R.p));
     }
     const float delta_p = R.p - L.p;
-    const float ubar_n =
-        fmaf(L.r, fabsf(u_L), R.r * fabsf(u_R)) * __builtin_amdgcn_rcpf(L.r + R.r);
+    const float ubar_n = fmaf(L.r, fabsf(u_L), R.r * fabsf(u_R)) * irS;
     const float g = wsat(-M_L) * wsat(M_R);
     const float ubp = fmaf(g, fabsf(u_L) - ubar_n, ubar_n);
     const float ubm = fmaf(g, fabsf(u_R) - ubar_n, ubar_n);
