@@ -17,20 +17,20 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // wave_shr:1 gives lane i the value of lane i-1, wave_shl:1 that of lane
 // i+1.  Edge lanes receive 0; they are halo lanes whose results are unused.
 __device__ __forceinline__ float from_left(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 // doubles move as two 32-bit halves
 __device__ __forceinline__ double from_left(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double from_right(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 template <typename R>
@@ -107,6 +107,32 @@ __device__ __forceinline__ R cell_wave_speed(const DevParamsT<R>& P, R rho, R mu
 // increment added by an error-free TwoSum, so that increments below half an
 // ulp of hi accumulate in lo instead of being rounded away.  a.acc carries
 // K1, then K1 + K2, from stage to stage at the cell.
+// Row access: every row a wavefront touches is wave-uniform, so an element
+// is a uniform row pointer (scalar registers) plus the lane's 32-bit byte
+// offset of its column, the saddr + voffset form of a global access (no
+// per-access 64-bit address arithmetic in vector registers).
+template <typename T>
+__device__ __forceinline__ T ld_row(const T* row, uint32_t boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(row) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_row(T* row, uint32_t boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(row) + boff) = v;
+}
+// the four planes of one state row at one lane
+template <typename R>
+__device__ __forceinline__ ConsT<R> ld_cons(const R* row, size_t PS, uint32_t boff) {
+  return ConsT<R>{ld_row(row, boff), ld_row(row + PS, boff), ld_row(row + 2 * PS, boff),
+                  ld_row(row + 3 * PS, boff)};
+}
+template <typename R>
+__device__ __forceinline__ void st_cons(R* row, size_t PS, uint32_t boff, R a, R b, R c, R d) {
+  st_row(row, boff, a);
+  st_row(row + PS, boff, b);
+  st_row(row + 2 * PS, boff, c);
+  st_row(row + 3 * PS, boff, d);
+}
+
 template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false>
 __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
                                         int tyb, R dt) {
@@ -115,14 +141,18 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
   using Cons = ConsT<R>;
   using Flux = FluxT<R>;
   const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
-  const int M = a.M, N = a.N, Pp = a.P, PS = a.PS;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int M = a.M, N = a.N, Pp = a.P;
+  const size_t PS = (size_t)a.PS;
   const int c0 = tx * kOwn;
   const int own = min(kOwn, M - c0);
   int col = (c0 - 2 + lane) % M;
   if (col < 0) col += M;
   const int colr = (col + 1 == M) ? 0 : col + 1;
   const bool owned = lane >= 2 && lane < 2 + own;
+  const uint32_t bc = (uint32_t)col * (uint32_t)sizeof(R);    // this lane's column
+  const uint32_t bcr = (uint32_t)colr * (uint32_t)sizeof(R);  // and its right neighbour
+  auto row_ptr = [&](const R* base, int j) { return base + (size_t)j * (size_t)Pp; };
   const int j0 = (tyb * kWavesPerBlock + wid) * TY;
   const int lim = a.limiter;
   const int stage = a.stage;
@@ -140,22 +170,20 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int j = min(max(j0 - 2 + k, 0), N - 1);
-      const int o = j * Pp + col;
-      raw[k] = Cons{qin[o], qin[PS + o], qin[2 * PS + o], qin[3 * PS + o]};
+      raw[k] = ld_cons(row_ptr(qin, j), PS, bc);
     }
     R vxr[TY + 1], vyr[TY + 1];
 #pragma unroll
     for (int k = 0; k <= TY; ++k) {
-      const int o = min(j0 + k, N) * Pp + col;
-      vxr[k] = a.vx[o];
-      vyr[k] = a.vy[o];
+      const int j = min(j0 + k, N);
+      vxr[k] = ld_row(row_ptr(a.vx, j), bc);
+      vyr[k] = ld_row(row_ptr(a.vy, j), bc);
     }
     Cons qnr[TY];
     if (stage >= 2) {
 #pragma unroll
       for (int k = 0; k < TY; ++k) {
-        const int o = min(j0 + k, N - 1) * Pp + col;
-        qnr[k] = Cons{a.qn[o], a.qn[PS + o], a.qn[2 * PS + o], a.qn[3 * PS + o]};
+        qnr[k] = ld_cons(row_ptr(a.qn, min(j0 + k, N - 1)), PS, bc);
       }
     }
     // compensated stages: the increment accumulator (stages 2, 3) and the
@@ -165,9 +193,9 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       if (stage >= 2) {
 #pragma unroll
         for (int k = 0; k < TY; ++k) {
-          const int o = min(j0 + k, N - 1) * Pp + col;
-          accr[k] = Cons{a.acc[o], a.acc[PS + o], a.acc[2 * PS + o], a.acc[3 * PS + o]};
-          if (stage == 3) lor[k] = Cons{a.lo[o], a.lo[PS + o], a.lo[2 * PS + o], a.lo[3 * PS + o]};
+          const int j = min(j0 + k, N - 1);
+          accr[k] = ld_cons(row_ptr(a.acc, j), PS, bc);
+          if (stage == 3) lor[k] = ld_cons(row_ptr(a.lo, j), PS, bc);
         }
       }
     }
@@ -178,20 +206,20 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       if constexpr (FAST) return to_prim_fast(q, P.gm1);
       else return to_prim_exact(q, P.gamma);
     };
-    auto row_of = [&](int j) -> Cons {
-      const int o = j * Pp + col;
-      return Cons{qin[o], qin[PS + o], qin[2 * PS + o], qin[3 * PS + o]};
-    };
+    auto row_of = [&](int j) -> Cons { return ld_cons(row_ptr(qin, j), PS, bc); };
     Prim gW{}, gF1{}, gF2{};
     if (j0 < 2) {
       R nx, ny, len;
-      normal_exact(a.vx[col] - a.vx[colr], a.vy[col] - a.vy[colr], nx, ny, len);
+      normal_exact(ld_row(a.vx, bc) - ld_row(a.vx, bcr), ld_row(a.vy, bc) - ld_row(a.vy, bcr), nx, ny,
+                   len);
       gW = to_prim(wall_ghost(row_of(0), nx, ny));
     }
     if (j0 + TY + 2 > N) {
-      const int o = N * Pp;
+      const R* vxN = row_ptr(a.vx, N);
+      const R* vyN = row_ptr(a.vy, N);
       R nx, ny, len;
-      normal_exact(a.vx[o + col] - a.vx[o + colr], a.vy[o + col] - a.vy[o + colr], nx, ny, len);
+      normal_exact(ld_row(vxN, bc) - ld_row(vxN, bcr), ld_row(vyN, bc) - ld_row(vyN, bcr), nx, ny,
+                   len);
       const Cons I = row_of(N - 1);
       gF1 = to_prim(far_ghost(P, I, nx, ny, false));
       gF2 = to_prim(far_ghost(P, I, nx, ny, true));
@@ -274,7 +302,6 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       }
 
       if (owned) {
-        const int o = r * Pp + col;
         R nq[4];
         const R cq[4] = {raw[t + 2].r, raw[t + 2].mu, raw[t + 2].mv, raw[t + 2].e};
         R dt_c = dt;
@@ -326,10 +353,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
             }
           }
           R* dst = stage == 3 ? a.lo : a.acc;
-          dst[o] = an[0];
-          dst[PS + o] = an[1];
-          dst[2 * PS + o] = an[2];
-          dst[3 * PS + o] = an[3];
+          st_cons(dst + (size_t)r * (size_t)Pp, PS, bc, an[0], an[1], an[2], an[3]);
         } else if (stage == 1) {
           // Q1 = Qn + dt L(Qn)  (ausm.js:463)
           for (int k = 0; k < 4; ++k) nq[k] = cq[k] + dt_c * res[k];
@@ -348,10 +372,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
             }
           }
         }
-        a.qout[o] = nq[0];
-        a.qout[PS + o] = nq[1];
-        a.qout[2 * PS + o] = nq[2];
-        a.qout[3 * PS + o] = nq[3];
+        st_cons(a.qout + (size_t)r * (size_t)Pp, PS, bc, nq[0], nq[1], nq[2], nq[3]);
         if (stage == 3 && a.refresh) {
           const R ws = cell_wave_speed<FAST>(P, nq[0], nq[1], nq[2], nq[3],
                                              ((lL + lR) + lU) + lD, area);
