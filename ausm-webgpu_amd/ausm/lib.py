@@ -72,6 +72,10 @@ SIGNATURES = {
     "ausm_boundary_naca4": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _vp]),
     "ausm_boundary_cst": (_i, [_vp, _vp, _i, _d, _d, _d, _d, _i, _i, _vp]),
     "ausm_boundary_circle": (_i, [_d, _i, _vp]),
+    "ausm_boundary_sears_haack": (_i, [_d, _d, _d, _i, _vp]),
+    "ausm_boundary_haack_projectile": (_i, [_d, _d, _d, _d, _d, _d, _d, _i, _vp]),
+    "ausm_boundary_polygon": (_i, [_d, _i, _d, _d, _d, _i, _vp]),
+    "ausm_boundary_dat": (_i, [ctypes.c_char_p, _d, _d, _i, _vp, ctypes.POINTER(ctypes.c_int)]),
     "ausm_boundary_rect": (_i, [_d, _i, _vp]),
     "ausm_grid_from_boundary": (_i, [_vp, _vp, _vp, _i, _i, _d]),
     "ausm_grid_set_vertices": (_i, [_vp, _vp]),

@@ -15,6 +15,7 @@ import numpy as np
 
 from . import lib as L
 from .lib import check, f32, f64, ptr
+from .lib import load as _load_lib
 
 
 def naca4_digits(code):
@@ -50,6 +51,40 @@ def cst_boundary(M, a_upper, a_lower, te_gap=0.0, size=0.5, x_offset=0.0, expone
     check(L.load().ausm_boundary_cst(ptr(au), ptr(al), au.size, te_gap, size, x_offset, exponent,
                                      sp, M, ptr(out)))
     return out
+
+
+def sears_haack_boundary(M, V=0.0002, L=0.4, x_offset=-0.7):
+    """Sears-Haack body of volume V and length L (src/grid.js:53-61; defaults
+    are the reference's object menu, src/grid.js:262)."""
+    out = np.zeros((M, 2), dtype=np.float64)
+    check(_load_lib().ausm_boundary_sears_haack(V, L, x_offset, M, ptr(out)))
+    return out
+
+
+def haack_projectile_boundary(M, F=6.0, R=0.03, L=0.5, Rb=0.5, Lb=0.2, C=0.0, x_offset=-0.7):
+    """Haack-series nose cone on a body with a boat tail (src/grid.js:75-118)."""
+    out = np.zeros((M, 2), dtype=np.float64)
+    check(_load_lib().ausm_boundary_haack_projectile(F, R, L, Rb, Lb, C, x_offset, M, ptr(out)))
+    return out
+
+
+def polygon_boundary(M, R=0.1, N=4, F=0.1, a=0.0, x_offset=-0.7):
+    """Regular N-gon body (src/grid.js:224-231); the reference's circle body
+    is N = 400, F = 1 (src/grid.js:258-259)."""
+    out = np.zeros((M, 2), dtype=np.float64)
+    check(L.load().ausm_boundary_polygon(R, N, F, a, x_offset, M, ptr(out)))
+    return out
+
+
+def dat_boundary(M, text, size=0.3, x_offset=-0.7):
+    """Airfoil coordinate file (Selig or Lednicer) resampled to M points
+    (src/airfoilDatParser.js); returns (points, format)."""
+    if isinstance(text, str):
+        text = text.encode()
+    out = np.zeros((M, 2), dtype=np.float64)
+    fmt = ctypes.c_int(-1)
+    check(L.load().ausm_boundary_dat(text, size, x_offset, M, ptr(out), ctypes.byref(fmt)))
+    return out, ("selig", "lednicer")[fmt.value]
 
 
 def circle_boundary(M, radius):

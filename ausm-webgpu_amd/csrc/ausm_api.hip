@@ -364,121 +364,6 @@ int ausm_sync(ausm_ctx* c) {
   return AUSM_OK;
 }
 
-// ---- geometry ---------------------------------------------------------------
-int ausm_boundary_naca4(double m, double p, double t, double size, double x_offset,
-                        double exponent, int spacing, int M, double* xy) {
-  if (!xy || M < 4) return set_error(AUSM_ERR_ARG, "bad NACA arguments");
-  if (m > 0 && (p <= 0 || p >= 1)) return set_error(AUSM_ERR_ARG, "camber position p must be in (0,1)");
-  const double chord = 2.0 * size;
-  for (int i = 0; i < M; ++i) {
-    // generateNACA4Boundary (grid.js:38-52): parameter s = i / (M/2)
-    const double s = (double)i / ((double)M / 2.0);
-    const bool upper = s <= 1.0;
-    double xc;
-    if (spacing == AUSM_SPACING_COSINE)
-      xc = upper ? 0.5 * (1.0 + std::cos(M_PI * s)) : 0.5 * (1.0 - std::cos(M_PI * (s - 1.0)));
-    else
-      xc = upper ? std::pow(1.0 - s, exponent) : std::pow(s - 1.0, exponent);
-    const double yt = 5.0 * t *
-                      (0.2969 * std::sqrt(xc) - 0.126 * xc - 0.3516 * xc * xc +
-                       0.2843 * xc * xc * xc - 0.1036 * xc * xc * xc * xc);
-    double yc = 0.0, dyc = 0.0;
-    if (m > 0) {
-      if (xc < p) {
-        yc = (m / (p * p)) * (2.0 * p * xc - xc * xc);
-        dyc = (m / (p * p)) * (2.0 * p - 2.0 * xc);
-      } else {
-        yc = (m / ((1.0 - p) * (1.0 - p))) * ((1.0 - 2.0 * p) + 2.0 * p * xc - xc * xc);
-        dyc = (m / ((1.0 - p) * (1.0 - p))) * (2.0 * p - 2.0 * xc);
-      }
-    }
-    // thickness applied perpendicular to the mean line
-    const double th = std::atan(dyc);
-    const double sx = yt * -std::sin(th), sy = yt / std::sqrt(1.0 + dyc * dyc);
-    const double sgn = upper ? 1.0 : -1.0;
-    xy[2 * i] = (xc - 0.5 + sgn * sx) * chord + x_offset;
-    xy[2 * i + 1] = (yc + sgn * sy) * chord;
-  }
-  return AUSM_OK;
-}
-
-// Class-shape transformation section (Kulfan): with n = n_coef - 1,
-//   y_u(x) = sqrt(x) (1 - x) sum_i A_u,i C(n,i) x^i (1-x)^(n-i) + x te_gap / 2
-//   y_l(x) = sqrt(x) (1 - x) sum_i A_l,i C(n,i) x^i (1-x)^(n-i) - x te_gap / 2
-// at the chordwise stations of the NACA generator (same point order).
-static double cst_surface(const double* a, int n_coef, double x) {
-  const int n = n_coef - 1;
-  double s = 0.0, binom = 1.0;
-  for (int i = 0; i <= n; ++i) {
-    double xi = 1.0, xr = 1.0;
-    for (int k = 0; k < i; ++k) xi *= x;
-    for (int k = 0; k < n - i; ++k) xr *= 1.0 - x;
-    s += a[i] * binom * xi * xr;
-    binom = binom * (double)(n - i) / (double)(i + 1);
-  }
-  return std::sqrt(x) * (1.0 - x) * s;
-}
-
-int ausm_boundary_cst(const double* a_upper, const double* a_lower, int n_coef, double te_gap,
-                      double size, double x_offset, double exponent, int spacing, int M,
-                      double* xy) {
-  if (!xy || !a_upper || !a_lower || M < 4) return set_error(AUSM_ERR_ARG, "bad CST arguments");
-  if (n_coef < 1 || n_coef > 32) return set_error(AUSM_ERR_ARG, "CST needs 1..32 coefficients per side");
-  const double chord = 2.0 * size;
-  for (int i = 0; i < M; ++i) {
-    const double s = (double)i / ((double)M / 2.0);
-    const bool upper = s <= 1.0;
-    double xc;
-    if (spacing == AUSM_SPACING_COSINE)
-      xc = upper ? 0.5 * (1.0 + std::cos(M_PI * s)) : 0.5 * (1.0 - std::cos(M_PI * (s - 1.0)));
-    else
-      xc = upper ? std::pow(1.0 - s, exponent) : std::pow(s - 1.0, exponent);
-    const double y = upper ? cst_surface(a_upper, n_coef, xc) + xc * (0.5 * te_gap)
-                           : cst_surface(a_lower, n_coef, xc) - xc * (0.5 * te_gap);
-    xy[2 * i] = (xc - 0.5) * chord + x_offset;
-    xy[2 * i + 1] = y * chord;
-  }
-  return AUSM_OK;
-}
-
-int ausm_boundary_circle(double radius, int M, double* xy) {
-  if (!xy || M < 4 || !(radius > 0)) return set_error(AUSM_ERR_ARG, "bad circle arguments");
-  for (int i = 0; i < M; ++i) {
-    const double t = (double)i / (double)M;  // grid.js:232-237
-    xy[2 * i] = radius * std::cos(t * M_PI * 2.0);
-    xy[2 * i + 1] = radius * std::sin(t * M_PI * 2.0);
-  }
-  return AUSM_OK;
-}
-
-int ausm_boundary_rect(double ratio, int M, double* xy) {
-  if (!xy || M < 4 || !(ratio > 0)) return set_error(AUSM_ERR_ARG, "bad rectangle arguments");
-  for (int i = 0; i < M; ++i) {
-    double t = std::fmod((double)i / (double)M + 0.125, 1.0);  // grid.js:238-255
-    double x, y;
-    if (t < 0.25) {
-      t *= 4.0;
-      x = ratio;
-      y = t * 2.0 - 1.0;
-    } else if (t < 0.5) {
-      t = (t - 0.25) * 4.0;
-      x = ((1.0 - t) * 2.0 - 1.0) * ratio;
-      y = 1.0;
-    } else if (t < 0.75) {
-      t = (t - 0.5) * 4.0;
-      x = -ratio;
-      y = (1.0 - t) * 2.0 - 1.0;
-    } else {
-      t = (t - 0.75) * 4.0;
-      x = (t * 2.0 - 1.0) * ratio;
-      y = -1.0;
-    }
-    xy[2 * i] = x;
-    xy[2 * i + 1] = y;
-  }
-  return AUSM_OK;
-}
-
 // ---- grid -------------------------------------------------------------------
 static int finish_grid(ausm_ctx* c, const double* gx, const double* gy) {
   const size_t nv = (size_t)(c->N + 1) * c->P;

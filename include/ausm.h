@@ -108,6 +108,26 @@ int ausm_boundary_naca4(double m, double p, double t, double size, double x_offs
 int ausm_boundary_cst(const double* a_upper, const double* a_lower, int n_coef, double te_gap,
                       double size, double x_offset, double exponent, int spacing, int M,
                       double* xy);
+/* Bodies the reference's object menu offers (src/grid.js:258-263), in the
+ * point order of ausm_boundary_naca4:
+ *   Sears-Haack body of volume V and length L (generateSearsHaackBoundary,
+ *   src/grid.js:53-61); Haack-series projectile: nose fineness F, radius R,
+ *   body length L (fraction of the nose length), boat-tail radius and
+ *   length fractions Rb, Lb, Haack parameter C (generateHaackProjectileBoundary,
+ *   src/grid.js:75-118); regular N-gon of inradius R rotated by a and
+ *   squashed by F in y, the reference's circle being N = 400, F = 1
+ *   (generatePolygonBoundary, src/grid.js:224-231). */
+int ausm_boundary_sears_haack(double V, double L, double x_offset, int M, double* xy);
+int ausm_boundary_haack_projectile(double F, double R, double L, double Rb, double Lb, double C,
+                                   double x_offset, int M, double* xy);
+int ausm_boundary_polygon(double R, int N, double F, double a, double x_offset, int M, double* xy);
+/* Airfoil coordinate file (Selig or Lednicer text, auto-detected) sampled
+ * at M points with a centripetal Catmull-Rom spline over arc length, denser
+ * at the nose; chord 2*size at x_offset (parseDat + sampleAirfoil,
+ * src/airfoilDatParser.js:67-293).  format (optional) gets 0 = Selig,
+ * 1 = Lednicer. */
+int ausm_boundary_dat(const char* text, double size, double x_offset, int M, double* xy,
+                      int* format);
 int ausm_boundary_circle(double radius, int M, double* xy);
 int ausm_boundary_rect(double length_ratio, int M, double* xy);
 

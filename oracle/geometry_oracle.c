@@ -104,3 +104,69 @@ void geo_rect_point(double ratio, int M, int i, double* xy) {
     xy[1] = -1.0;
   }
 }
+
+/* generateSearsHaackBoundary (src/grid.js:53-61) for vertex i of M. */
+void geo_sears_haack_point(double V, double L, double x_offset, int M, int i, double* xy) {
+  double t = (double)i / ((double)M / 2.0);
+  const double c = 8.0 / M_PI * sqrt(2.0 * V / (3.0 * L));
+  if (t < 1.0) {
+    xy[0] = -L * t + x_offset;
+    xy[1] = c * pow(t * (1.0 - t), 0.75);
+  } else {
+    t = t - 1.0;
+    xy[0] = L * (t - 1.0) + x_offset;
+    xy[1] = -c * pow(t * (1.0 - t), 0.75);
+  }
+}
+
+/* generateHaackProjectileBoundary (src/grid.js:75-118) for vertex i of M:
+ * tail cap, boat tail, cylinder and Haack nose on the upper side (TE to
+ * nose), mirrored on the lower side. */
+static double haack_r(double R, double C, double T) {
+  const double s = sin(T);
+  return R * sqrt((T - 0.5 * sin(2.0 * T) + C / 3.0 * s * s * s) / M_PI);
+}
+
+void geo_haack_point(double F, double R, double L, double Rb, double Lb, double C, double x_offset,
+                     int M, int i, double* xy) {
+  const double FR = F * R;
+  const double cone = R * sqrt(1.0 + 4.0 * F * F);
+  const double body = FR * L * (1.0 - Lb);
+  const double btx = FR * L * Lb;
+  const double bt = sqrt(btx * btx + (1.0 - Rb) * (1.0 - Rb) * R * R);
+  const double slope = -(1.0 - Rb) * R / btx;
+  const double tail = R * Rb;
+  const double tot = cone + body + bt + tail;
+  const double f1 = tail / tot, f2 = bt / tot + f1, f3 = body / tot + f2;
+  double t = (double)i / ((double)M / 2.0);
+  const int up = t < 1.0;
+  /* position along the upper traversal (TE -> nose) */
+  double u = up ? t : 1.0 - (t - 1.0);
+  double x, y;
+  if (up ? (u < f1) : (u <= f1)) {
+    const double s = u / f1;
+    x = FR * L; y = R * Rb * s;
+  } else if (up ? (u < f2) : (u <= f2)) {
+    const double s = (u - f1) / (f2 - f1);
+    x = FR * L - btx * s; y = R + slope * (x - body);
+  } else if (up ? (u < f3) : (u <= f3)) {
+    const double s = (u - f2) / (f3 - f2);
+    x = body * (1.0 - s); y = R;
+  } else {
+    const double s = (u - f3) / (1.0 - f3);
+    x = -2.0 * FR * s; y = haack_r(R, C, acos(2.0 * s - 1.0));
+  }
+  xy[0] = x + x_offset;
+  xy[1] = up ? y : -y;
+}
+
+/* generatePolygonBoundary (src/grid.js:224-231) for vertex i of M. */
+void geo_polygon_point(double R, int N, double F, double a, double x_offset, int M, int i,
+                       double* xy) {
+  const double t = 2.0 * M_PI * (double)i / (double)M;
+  const double b = 2.0 * M_PI / (double)N;
+  const double ph = fmod(t + a + b, b) - M_PI / (double)N;
+  const double r = R / cos(ph);
+  xy[0] = r * cos(t) + x_offset;
+  xy[1] = F * r * sin(t);
+}

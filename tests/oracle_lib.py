@@ -97,6 +97,9 @@ def _declare(L, prefixes=(("orc32_", ctypes.c_float), ("orc64_", ctypes.c_double
         getattr(L, n).argtypes = [cd, cd, cd, cd, cd, cd, ci, ci, ci, vp]
     L.geo_circle_point.argtypes = [cd, ci, ci, vp]
     L.geo_cst_point.argtypes = [vp, vp, ci, cd, cd, cd, cd, ci, ci, ci, vp]
+    L.geo_sears_haack_point.argtypes = [cd, cd, cd, ci, ci, vp]
+    L.geo_haack_point.argtypes = [cd, cd, cd, cd, cd, cd, cd, ci, ci, vp]
+    L.geo_polygon_point.argtypes = [cd, ci, cd, cd, cd, ci, ci, vp]
     L.geo_rect_point.argtypes = [cd, ci, ci, vp]
 
 
@@ -163,6 +166,25 @@ def cst_boundary(M, a_upper, a_lower, te_gap=0.0, size=0.5, x_offset=0.0, expo=1
         lib().geo_cst_point(_ptr(au), _ptr(al), au.size, te_gap, size, x_offset, expo, spacing, M,
                             i, _ptr(out[i]))
     return out
+
+
+def _points(M, fn, *args):
+    out = np.zeros((M, 2), dtype=np.float64)
+    for i in range(M):
+        fn(*args, M, i, _ptr(out[i]))
+    return out
+
+
+def sears_haack_boundary(M, V, L, x_offset):
+    return _points(M, lib().geo_sears_haack_point, V, L, x_offset)
+
+
+def haack_boundary(M, F=6.0, R=0.03, L=0.5, Rb=0.5, Lb=0.2, C=0.0, x_offset=-0.7):
+    return _points(M, lib().geo_haack_point, F, R, L, Rb, Lb, C, x_offset)
+
+
+def polygon_boundary(M, R, N, F, a, x_offset):
+    return _points(M, lib().geo_polygon_point, R, N, F, a, x_offset)
 
 
 def naca4_digits(code):
