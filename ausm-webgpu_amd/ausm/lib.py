@@ -17,6 +17,9 @@ VANALBADA, MINMOD, VANLEER, NOLIMITER = 0, 1, 2, 3
 MATH_EXACT, MATH_FAST = 0, 1
 PREC_FP32, PREC_FP64, PREC_FP32C = 0, 1, 2
 DT_GLOBAL, DT_LOCAL = 0, 1
+# derived display fields (the reference's simDisplayMode numbers)
+FIELDS = {"schlieren": 0, "vorticity": 1, "density": 2, "temperature": 3, "pressure": 4,
+          "mach": 5, "mach_inflow": 6, "velocity_rel": 7, "entropy": 8, "total_pressure": 9}
 INIT_REST, INIT_FREESTREAM = 0, 1
 GRID_JACOBI, GRID_SOR = 0, 1
 SPACING_POWER, SPACING_COSINE = 0, 1
@@ -98,6 +101,7 @@ SIGNATURES = {
     "ausm_wave_speeds": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
     "ausm_forces": (_i, [_vp, _d, _vp, _vp]),
     "ausm_residual_norm": (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "ausm_derived_field": (_i, [_vp, _i, _vp, ctypes.POINTER(ctypes.c_float)]),
     "ausm_set_state64": (_i, [_vp, _vp]),
     "ausm_get_state64": (_i, [_vp, _vp]),
     "ausm_eval_residual64": (_i, [_vp, _vp]),

@@ -306,6 +306,17 @@ class Solver:
             res["cp"] = cp
         return res
 
+    def derived_field(self, kind):
+        """Display field of the current solution (the visualization pass,
+        ausm.js:586-710, before its colour map): kind is a name of
+        lib.FIELDS or its number.  Returns (N x M float32, free-stream value)."""
+        if isinstance(kind, str):
+            kind = L.FIELDS[kind]
+        out = np.zeros((self.N, self.M), dtype=np.float32)
+        fs = ctypes.c_float()
+        check(self._L.ausm_derived_field(self.h, kind, ptr(out), ctypes.byref(fs)))
+        return out, fs.value
+
     def residual_norm(self):
         v = ctypes.c_double()
         check(self._L.ausm_residual_norm(self.h, ctypes.byref(v)))

@@ -941,6 +941,24 @@ int ausm_forces(ausm_ctx* c, double ref_length, double* out, double* cp) {
   return AUSM_OK;
 }
 
+int ausm_derived_field(ausm_ctx* c, int kind, float* out, float* freestream_value) {
+  if (!c || !out) return set_error(AUSM_ERR_ARG, "NULL argument");
+  if (kind < 0 || kind > 9) return set_error(AUSM_ERR_ARG, "unknown field %d", kind);
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");
+  hipSetDevice(c->device);
+  float* dout = c->stage_buf;                       // N x M
+  float* dfs = c->stage_buf + (size_t)c->M * c->N;  // one float after it
+  if (fp64(c))
+    CHECK_HIP(launch_derived_field(c->q64[0], c->dpar64, c->M, c->N, c->P, c->PS, kind, dout, dfs, c->stream), "derived field");
+  else
+    CHECK_HIP(launch_derived_field(c->q[0], c->dpar, c->M, c->N, c->P, c->PS, kind, dout, dfs, c->stream), "derived field");
+  CHECK_HIP(hipMemcpyAsync(out, dout, sizeof(float) * (size_t)c->M * c->N, hipMemcpyDeviceToHost, c->stream), "derived field");
+  CHECK_HIP(hipMemcpyAsync(c->pinned, dfs, sizeof(float), hipMemcpyDeviceToHost, c->stream), "derived field");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "derived field");
+  if (freestream_value) *freestream_value = *(float*)c->pinned;
+  return AUSM_OK;
+}
+
 int ausm_residual_norm(ausm_ctx* c, double* l2) {
   if (!c || !l2) return set_error(AUSM_ERR_ARG, "NULL argument");
   if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state");

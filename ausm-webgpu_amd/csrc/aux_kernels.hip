@@ -351,6 +351,91 @@ __global__ void convert_planes_kernel(const A* a, B* b, int M, int N, int P, int
 }
 
 // --------------------------------------------------------------------------
+// derived display fields (visualization pass, ausm.js:586-710): the scalar
+// before the colour map; mode = the pass's simDisplayMode (schlieren,
+// vorticity, density, temperature, pressure, Mach, |v - V_inf|, entropy,
+// total pressure).  Block 0's thread 0 also writes the free-stream value.
+// --------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R dexp(R x) { if constexpr (sizeof(R) == 4) return expf(x); else return exp(x); }
+template <typename R>
+__device__ __forceinline__ R dlog(R x) { if constexpr (sizeof(R) == 4) return logf(x); else return log(x); }
+template <typename R>
+__device__ __forceinline__ R dpow(R x, R y) { if constexpr (sizeof(R) == 4) return powf(x, y); else return pow(x, y); }
+
+template <typename R>
+__global__ void derived_field_kernel(const R* q, const DevParamsT<R>* par, int M, int N, int P,
+                                     int PS, int mode, float* out, float* fs_out) {
+  const R gamma = par->gamma;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    const R inVel = sqrt(par->vx * par->vx + par->vy * par->vy);
+    const R inTemp = par->inP / par->inRho;
+    const R inMach = inVel / sqrt(gamma * inTemp);
+    R fs;
+    switch (mode) {
+      case 0: fs = R(1); break;
+      case 1: case 7: fs = R(0); break;
+      case 3: fs = inTemp; break;
+      case 4: fs = par->inP; break;
+      case 5: fs = R(0.5); break;
+      case 6: fs = inMach; break;
+      case 8: fs = dlog(par->inP / dpow(par->inRho, gamma)); break;
+      case 9: fs = par->inP * dpow(R(1) + R(0.5) * (gamma - R(1)) * inMach * inMach, gamma / (gamma - R(1))); break;
+      default: fs = par->inRho;
+    }
+    *fs_out = (float)fs;
+  }
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= M || y >= N) return;
+  auto ld = [&](int j, int i) {
+    const int o = j * P + i;
+    return ConsT<R>{q[o], q[PS + o], q[2 * PS + o], q[3 * PS + o]};
+  };
+  const ConsT<R> c = ld(y, x);
+  const R rho = c.r;
+  const R vx = c.mu / rho, vy = c.mv / rho;
+  const R vv = vx * vx + vy * vy;
+  const R p = (c.e - (R(0.5) * rho) * vv) * (gamma - R(1));
+  const R T = p / rho;
+  const R a = sqrt(gamma * T);
+  const R speed = sqrt(vv);
+  R v;
+  if (mode <= 1) {
+    const ConsT<R> L = ld(y, x == 0 ? M - 1 : x - 1);
+    const ConsT<R> Rr = ld(y, x + 1 == M ? 0 : x + 1);
+    const ConsT<R> D = ld(max(y - 1, 0), x);
+    const ConsT<R> U = ld(min(y + 1, N - 2), x);
+    if (mode == 0) {
+      const R gx = Rr.r - L.r, gy = U.r - D.r;
+      v = dexp(-sqrt(gx * gx + gy * gy) * R(2));
+    } else {
+      v = (Rr.mv / Rr.r - L.mv / L.r) - (U.mu / U.r - D.mu / D.r);
+    }
+  } else {
+    switch (mode) {
+      case 3: v = T; break;
+      case 4: v = p; break;
+      case 5: case 6: v = speed / a; break;
+      case 7: {
+        const R dx = vx - par->vx, dy = vy - par->vy;
+        v = sqrt(dx * dx + dy * dy);
+        break;
+      }
+      case 8: v = dlog(p / dpow(rho, gamma)); break;
+      case 9: {
+        const R m = speed / a;
+        v = p * dpow(R(1) + R(0.5) * (gamma - R(1)) * m * m, gamma / (gamma - R(1)));
+        break;
+      }
+      default: v = rho;
+    }
+  }
+  out[(size_t)y * M + x] = (float)v;
+}
+
+// --------------------------------------------------------------------------
+// launchers// --------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------
 static dim3 rows_grid(int M, int rows) { return dim3((M + 255) / 256, rows); }
@@ -447,6 +532,18 @@ hipError_t launch_convert_planes(const A* a, B* b, int M, int N, int P, int PS, 
                      PS);
   return hipGetLastError();
 }
+
+template <typename R>
+hipError_t launch_derived_field(const R* q, const DevParamsT<R>* par, int M, int N, int P, int PS,
+                                int mode, float* out, float* fs_out, hipStream_t s) {
+  hipLaunchKernelGGL(derived_field_kernel<R>, rows_grid(M, N), dim3(256), 0, s, q, par, M, N, P, PS,
+                     mode, out, fs_out);
+  return hipGetLastError();
+}
+template hipError_t launch_derived_field<float>(const float*, const DevParamsT<float>*, int, int, int,
+                                                int, int, float*, float*, hipStream_t);
+template hipError_t launch_derived_field<double>(const double*, const DevParamsT<double>*, int, int,
+                                                 int, int, int, float*, float*, hipStream_t);
 
 #define AUSM_AUX_INSTANTIATE(R)                                                                   \
   template hipError_t launch_wave_speed<R>(const R*, const R*, const R*, const DevParamsT<R>*, int, \

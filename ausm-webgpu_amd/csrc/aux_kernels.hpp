@@ -38,6 +38,11 @@ hipError_t launch_fill_state(R* q, const DevParamsT<R>* par, int M, int N, int P
                              int freestream, hipStream_t s);
 template <typename A, typename B>
 hipError_t launch_convert_planes(const A* a, B* b, int M, int N, int P, int PS, hipStream_t s);
+// derived display field of every interior cell (N x M floats) and its
+// free-stream value
+template <typename R>
+hipError_t launch_derived_field(const R* q, const DevParamsT<R>* par, int M, int N, int P, int PS,
+                                int mode, float* out, float* fs_out, hipStream_t s);
 // compensated pair hi + lo (exact in FP64) <-> FP64 state planes
 hipError_t launch_comp_to_f64(const float* hi, const float* lo, double* q, int M, int N, int P,
                               int PS, hipStream_t s);

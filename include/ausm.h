@@ -203,6 +203,21 @@ int ausm_wave_speeds(ausm_ctx* ctx, float* ws, float* max_ws);
  * (ref_length = 1 gives the reference's un-normalised values).  cp (M
  * values, optional) gets (p - p_inf)/q_inf per body segment. */
 int ausm_forces(ausm_ctx* ctx, double ref_length, double* out, double* cp);
+/* Derived display field of the current solution (the visualization pass,
+ * src/shaders/ausm.js:586-710, up to its colour map): kind = the pass's
+ * simDisplayMode, 0 schlieren exp(-2|grad rho|), 1 vorticity, 2 density,
+ * 3 temperature p/rho, 4 pressure, 5 Mach (6: the same field, free-stream
+ * value the inflow Mach instead of 0.5), 7 |v - V_inf| (the pass shows its
+ * components), 8 entropy log(p/rho^gamma), 9 total pressure.  out: N x M
+ * interior cells (row-major, i fastest); freestream_value (optional): the
+ * value the pass's colour map is centred on. */
+enum ausm_field {
+  AUSM_FIELD_SCHLIEREN = 0, AUSM_FIELD_VORTICITY = 1, AUSM_FIELD_DENSITY = 2,
+  AUSM_FIELD_TEMPERATURE = 3, AUSM_FIELD_PRESSURE = 4, AUSM_FIELD_MACH = 5,
+  AUSM_FIELD_MACH_INFLOW = 6, AUSM_FIELD_VELOCITY_REL = 7, AUSM_FIELD_ENTROPY = 8,
+  AUSM_FIELD_TOTAL_PRESSURE = 9
+};
+int ausm_derived_field(ausm_ctx* ctx, int kind, float* out, float* freestream_value);
 /* FP64 L2 norm of the density residual of the current solution */
 int ausm_residual_norm(ausm_ctx* ctx, double* l2);
 /* FP64 state and residual, layouts as ausm_set_state / ausm_get_state /

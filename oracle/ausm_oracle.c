@@ -759,6 +759,94 @@ R FN(run_steps_comp)(const R* par, int scheme, int limiter, int M, int N, const 
   return dt;
 }
 
+/* Derived display fields of the visualization pass (ausm.js:586-710), the
+ * scalar before the colour map, for every interior cell (out N x M), and the
+ * free-stream value the colour map centres on (*fs).  mode = simDisplayMode:
+ * 0 schlieren exp(-2|grad rho|) with the pass's one-sided clamps (down row
+ * max(j-1, 0), up row min(j+1, N-2)), 1 vorticity dv/dx - du/dy by the same
+ * differences, 2 density, 3 temperature p/rho, 4 pressure, 5 and 6 Mach
+ * (free-stream value 0.5 or the inflow Mach), 7 |v - V_inf| (the pass shows
+ * its components), 8 entropy log(p / rho^gamma), 9 total pressure.
+ * Parity: modes 1-7 bitwise (division and square root only), 0, 8, 9 to a
+ * few ulp of exp / log / pow. */
+void FN(derived_field)(const R* par, int M, int N, const R* st, int mode, R* out, R* fs) {
+  const R gamma = par[P_GAMMA];
+  const R inRho = par[P_INRHO], inP = par[P_INP];
+  const R Vx = par[P_VX], Vy = par[P_VY];
+  const R inVel = rsqrt_(Vx * Vx + Vy * Vy);
+  const R inTemp = inP / inRho;
+  const R inMach = inVel / rsqrt_(gamma * inTemp);
+  const int fl = sizeof(R) == 4;
+  switch (mode) {
+    case 0: *fs = (R)1; break;
+    case 1: case 7: *fs = (R)0; break;
+    case 3: *fs = inTemp; break;
+    case 4: *fs = inP; break;
+    case 5: *fs = (R)0.5; break;
+    case 6: *fs = inMach; break;
+    case 8: *fs = fl ? (R)logf((float)(inP / (R)powf((float)inRho, (float)gamma)))
+                     : (R)log((double)(inP / (R)pow((double)inRho, (double)gamma))); break;
+    case 9: {
+      const R b = (R)1 + (R)0.5 * (gamma - (R)1) * inMach * inMach;
+      const R e = gamma / (gamma - (R)1);
+      *fs = inP * (fl ? (R)powf((float)b, (float)e) : (R)pow((double)b, (double)e));
+      break;
+    }
+    default: *fs = inRho;
+  }
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < M; ++x) {
+      const R* s = st + ((size_t)(y + 1) * M + x) * 4;
+      const R rho = s[0];
+      const R vx = s[1] / rho, vy = s[2] / rho;
+      const R vv = vx * vx + vy * vy;
+      const R p = (s[3] - ((R)0.5 * rho) * vv) * (gamma - (R)1);
+      const R T = p / rho;
+      const R a = rsqrt_(gamma * T);
+      const R speed = rsqrt_(vv);
+      R v;
+      if (mode <= 1) {
+        const R* L = st + ((size_t)(y + 1) * M + (x + M - 1) % M) * 4;
+        const R* Rr = st + ((size_t)(y + 1) * M + (x + 1) % M) * 4;
+        const int dn = y > 1 ? y : 1;                 /* texture rows */
+        const int upr = y + 2 < N - 1 ? y + 2 : N - 1;
+        const R* D = st + ((size_t)dn * M + x) * 4;
+        const R* U = st + ((size_t)upr * M + x) * 4;
+        if (mode == 0) {
+          const R gx = Rr[0] - L[0], gy = U[0] - D[0];
+          const R g = rsqrt_(gx * gx + gy * gy);
+          v = fl ? (R)expf((float)(-g * (R)2)) : (R)exp((double)(-g * (R)2));
+        } else {
+          v = (Rr[2] / Rr[0] - L[2] / L[0]) - (U[1] / U[0] - D[1] / D[0]);
+        }
+      } else {
+        switch (mode) {
+          case 3: v = T; break;
+          case 4: v = p; break;
+          case 5: case 6: v = speed / a; break;
+          case 7: {
+            const R dx = vx - Vx, dy = vy - Vy;
+            v = rsqrt_(dx * dx + dy * dy);
+            break;
+          }
+          case 8:
+            v = fl ? (R)logf((float)(p / (R)powf((float)rho, (float)gamma)))
+                   : (R)log((double)(p / (R)pow((double)rho, (double)gamma)));
+            break;
+          case 9: {
+            const R m = speed / a;
+            const R b = (R)1 + (R)0.5 * (gamma - (R)1) * m * m;
+            const R e = gamma / (gamma - (R)1);
+            v = p * (fl ? (R)powf((float)b, (float)e) : (R)pow((double)b, (double)e));
+            break;
+          }
+          default: v = rho;
+        }
+      }
+      out[(size_t)y * M + x] = v;
+    }
+}
+
 /* L2 norm of one residual component over the interior, accumulated in FP64
  * in index order. */
 double FN(residual_norm)(int M, int N, const R* res, int comp) {

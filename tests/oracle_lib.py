@@ -89,6 +89,7 @@ def _declare(L, prefixes=(("orc32_", ctypes.c_float), ("orc64_", ctypes.c_double
         g("run_steps_comp").argtypes = [vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci,
                                         vp, vp, ci]
         g("run_steps_comp").restype = cr
+        g("derived_field").argtypes = [vp, ci, ci, vp, ci, vp, vp]
         g("residual_norm").argtypes = [ci, ci, vp, ci]
         g("residual_norm").restype = cd
     if not geometry:
@@ -303,6 +304,16 @@ class Oracle:
                                      _ptr(self.arr(flen)), _ptr(self.arr(area)),
                                      _ptr(ws) if want_field else None)
         return (v, ws) if want_field else v
+
+    def derived_field(self, par, st, mode):
+        """(N x M field, free-stream value) of the visualization pass."""
+        st = self.arr(st)
+        N = st.shape[0] - 3
+        M = st.shape[1]
+        out = np.zeros((N, M), dtype=self.dt)
+        fs = np.zeros(1, dtype=self.dt)
+        self.f("derived_field")(_ptr(self.arr(par)), M, N, _ptr(st), mode, _ptr(out), _ptr(fs))
+        return out, fs[0]
 
     def forces(self, par, st, vtx, ref_len=1.0, want_cp=False):
         M = vtx.shape[1]
