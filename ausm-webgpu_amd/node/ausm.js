@@ -15,6 +15,8 @@ const limiters = Object.freeze({ vanAlbada: 0, minmod: 1, vanLeer: 2, none: 3 })
 const INIT_REST = 0;
 const INIT_FREESTREAM = 1;
 
+const now = () => Number(process.hrtime.bigint()) / 1e6;  // ms
+
 function toF64Pairs(coords) {
   if (coords instanceof Float64Array) return coords;
   const out = new Float64Array(coords.length * 2);
@@ -31,6 +33,38 @@ function naca4Boundary(M, naca4, size = 1.0, xOffset = 0, pointDistExponent = 7 
 }
 const circularOuterBoundary = (M, radius = 1) => native.boundaryCircle(radius, M);
 const rectangularOuterBoundary = (M, lengthRatio = 1.5) => native.boundaryRect(lengthRatio, M);
+// generatePolygonBoundary, generateSearsHaackBoundary, generateHaackProjectileBoundary
+// (src/grid.js:53-231) and sampleAirfoil (src/airfoilDatParser.js:276-293)
+const polygonBoundary = (M, R = 0.2, N = 4, F = 0.1, a = 0, xOffset = -0.7) =>
+  native.boundaryPolygon(R, N, F, a, xOffset, M);
+const searsHaackBoundary = (M, V = 0.2, L = 10, xOffset = -0.7) =>
+  native.boundarySearsHaack(V, L, xOffset, M);
+const haackProjectileBoundary = (M, F = 6, R = 0.03, L = 0.5, Rb = 0.5, Lb = 0.2, C = 0, xOffset = -0.7) =>
+  native.boundaryHaackProjectile(F, R, L, Rb, Lb, C, xOffset, M);
+const datBoundary = (M, datText, scale = 1.0, xOffset = 0.0) => native.boundaryDat(datText, scale, xOffset, M);
+const cstBoundary = (M, aUpper, aLower, teGap = 0, size = 0.5, xOffset = 0, exponent = 7 / 5, spacing = 0) =>
+  native.boundaryCst(Float64Array.from(aUpper), Float64Array.from(aLower), teGap, size, xOffset, exponent,
+                     spacing, M);
+
+// the object menu of the reference with its settings (src/grid.js:258-263)
+function objectBoundary(type, M, datText) {
+  switch (type) {
+    case 'polygon': return polygonBoundary(M, 0.1, 4, 0.1, 0, -0.7);
+    case 'circle': return polygonBoundary(M, 0.1, 400, 1, 0, -0.7);
+    case 'naca-4': return naca4Boundary(M, 4412, 0.2, -0.7);
+    case 'airfoil-dat': return datBoundary(M, datText, 0.3, -0.7);
+    case 'sears-haack': return searsHaackBoundary(M, 0.0002, 0.4, -0.7);
+    case 'haack-projectile': return haackProjectileBoundary(M);
+    default: throw new Error(`unknown object type ${type}`);
+  }
+}
+
+// display modes of the visualization pass (src/shaders/ausm.js:586-710)
+const fields = Object.freeze({
+  schlieren: 0, vorticity: 1, density: 2, temperature: 3, pressure: 4, mach: 5, machInflow: 6,
+  velocityRel: 7, entropy: 8, totalPressure: 9,
+});
+const precisions = Object.freeze({ fp32: 0, fp64: 1, fp32c: 2 });
 
 class AusmSolver {
   // simulationDomain = [M, N] (src/grid.js:12)
@@ -55,6 +89,12 @@ class AusmSolver {
     this.stepsPerFrame = 70;
     this.solver = 'SLAU2';
     this.mathMode = 0;
+    // adaptive steps per frame (main.js:720-731)
+    this.autoStepsPerFrame = false;
+    this.targetFrameTime = 1000 / 60;
+    this._lastFrameStart = 0;
+    this._lastTotal = 0;
+    this._oldDt = null;
     this._body = null;
     this._outer = null;
   }
@@ -101,22 +141,63 @@ class AusmSolver {
                      this.K_p, this.K_u);
   }
 
+  // body row from the reference's object menu (src/grid.js:258-263)
+  setObject(type, datText) { this._body = objectBoundary(type, this.M, datText); }
+
+  // play / pause and restart (src/util.js:192-201): pausing sets maxdt to 0,
+  // which render() treats as "do not step" (main.js:719)
+  togglePause() {
+    if (this._oldDt !== null) {
+      this.maxdt = this._oldDt;
+      this._oldDt = null;
+    } else {
+      this._oldDt = this.maxdt;
+      this.maxdt = 0;
+    }
+  }
+  get running() { return this.maxdt > 0; }
+  restart() { this.prepareState(); }
+
+  // adaptive stepsPerFrame (main.js:720-731): grow by one while the last
+  // frame's compute time stays 2 ms under the target, shrink by 5 % of the
+  // frame-interval overshoot (at most 100 ms)
+  _adapt(start) {
+    if (this.autoStepsPerFrame && this._lastTotal > 0 && this.running) {
+      const timeDifference = (start - this._lastFrameStart) - this.targetFrameTime;
+      if (this._lastTotal < this.targetFrameTime - 2) this.stepsPerFrame += 1;
+      this.stepsPerFrame = Math.max(1, Math.round(this.stepsPerFrame - Math.min(100, timeDifference) * 0.05));
+    }
+    this._lastFrameStart = start;
+  }
+
   // One frame of render() (src/main.js:735-784): upload the uniforms, take
   // stepsPerFrame steps with one dt for the frame, then ramp the inflow for
   // the next frame.
   frame() {
+    const start = now();
+    this._adapt(start);
     this._uploadUniforms();
-    const info = native.step(this.ctx, this.stepsPerFrame);
+    let info = null;
+    if (this.running) info = native.step(this.ctx, this.stepsPerFrame);
+    this._lastTotal = now() - start;
     this._ramp();
     return info;
   }
 
   async frameAsync() {
+    const start = now();
+    this._adapt(start);
     this._uploadUniforms();
-    const info = await native.stepAsync(this.ctx, this.stepsPerFrame);
+    let info = null;
+    if (this.running) info = await native.stepAsync(this.ctx, this.stepsPerFrame);
+    this._lastTotal = now() - start;
     this._ramp();
     return info;
   }
+
+  setPrecision(name) { native.setPrecision(this.ctx, precisions[name]); }
+  setDtMode(local) { native.setDtMode(this.ctx, local ? 1 : 0); }
+  field(kind) { return native.derivedField(this.ctx, typeof kind === 'string' ? fields[kind] : kind, this.M, this.N); }
 
   _ramp() {
     this.actualInflowVel += (this.inflowVel - this.actualInflowVel) /
@@ -134,6 +215,7 @@ class AusmSolver {
 }
 
 module.exports = {
-  AusmSolver, solvers, limiters, naca4Boundary, circularOuterBoundary, rectangularOuterBoundary,
-  INIT_REST, INIT_FREESTREAM, native,
+  AusmSolver, solvers, limiters, fields, precisions, naca4Boundary, circularOuterBoundary,
+  rectangularOuterBoundary, polygonBoundary, searsHaackBoundary, haackProjectileBoundary, datBoundary,
+  cstBoundary, objectBoundary, INIT_REST, INIT_FREESTREAM, native,
 };

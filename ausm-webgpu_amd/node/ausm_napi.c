@@ -145,6 +145,86 @@ static napi_value js_rect(napi_env env, napi_callback_info info) {
   return out;
 }
 
+static napi_value js_cst(napi_env env, napi_callback_info info) {
+  napi_value a[8];
+  if (!get_args(env, info, 8, a)) return NULL;
+  size_t nu = 0, nl = 0;
+  const double* au = (const double*)typed(env, a[0], napi_float64_array, &nu);
+  if (!au) return NULL;
+  const double* al = (const double*)typed(env, a[1], napi_float64_array, &nl);
+  if (!al) return NULL;
+  if (nu != nl) return throw_ausm(env, AUSM_ERR_ARG);
+  const int M = (int)num(env, a[7]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_cst(au, al, (int)nu, num(env, a[2]), num(env, a[3]), num(env, a[4]),
+                                   num(env, a[5]), (int)num(env, a[6]), M, (double*)d));
+  return out;
+}
+
+static napi_value js_sears_haack(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return NULL;
+  const int M = (int)num(env, a[3]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_sears_haack(num(env, a[0]), num(env, a[1]), num(env, a[2]), M, (double*)d));
+  return out;
+}
+
+static napi_value js_haack(napi_env env, napi_callback_info info) {
+  napi_value a[8];
+  if (!get_args(env, info, 8, a)) return NULL;
+  const int M = (int)num(env, a[7]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_haack_projectile(num(env, a[0]), num(env, a[1]), num(env, a[2]),
+                                                num(env, a[3]), num(env, a[4]), num(env, a[5]),
+                                                num(env, a[6]), M, (double*)d));
+  return out;
+}
+
+static napi_value js_polygon(napi_env env, napi_callback_info info) {
+  napi_value a[6];
+  if (!get_args(env, info, 6, a)) return NULL;
+  const int M = (int)num(env, a[5]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_polygon(num(env, a[0]), (int)num(env, a[1]), num(env, a[2]),
+                                       num(env, a[3]), num(env, a[4]), M, (double*)d));
+  return out;
+}
+
+/* boundaryDat(text, size, xOffset, M) -> Float64Array (2M) */
+static napi_value js_dat(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return NULL;
+  size_t len = 0;
+  NAPI_CALL(env, napi_get_value_string_utf8(env, a[0], NULL, 0, &len));
+  char* text = (char*)malloc(len + 1);
+  if (!text) return throw_ausm(env, AUSM_ERR_NOMEM);
+  if (napi_get_value_string_utf8(env, a[0], text, len + 1, &len) != napi_ok) {
+    free(text);
+    napi_throw_error(env, NULL, "expected a string");
+    return NULL;
+  }
+  const int M = (int)num(env, a[3]);
+  if (M < 4) {
+    free(text);
+    return throw_ausm(env, AUSM_ERR_ARG);
+  }
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  const int rc = ausm_boundary_dat(text, num(env, a[1]), num(env, a[2]), M, (double*)d, NULL);
+  free(text);
+  if (rc != AUSM_OK) return throw_ausm(env, rc);
+  return out;
+}
+
 /* ---- grid -------------------------------------------------------------------- */
 static napi_value js_grid_from_boundary(napi_env env, napi_callback_info info) {
   napi_value a[6];
@@ -357,6 +437,42 @@ static napi_value js_residual(napi_env env, napi_callback_info info) {
   return out;
 }
 
+static napi_value js_set_precision(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  AUSM_CALL(env, ausm_set_precision(c, (int)num(env, a[1])));
+  return NULL;
+}
+
+static napi_value js_set_dt_mode(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  AUSM_CALL(env, ausm_set_dt_mode(c, (int)num(env, a[1])));
+  return NULL;
+}
+
+/* derivedField(ctx, kind, M, N) -> {field: Float32Array (N*M), freestream} */
+static napi_value js_derived_field(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return NULL;
+  ausm_ctx* c = ctx_of(env, a[0]);
+  if (!c) return NULL;
+  const size_t n = (size_t)num(env, a[2]) * (size_t)num(env, a[3]);
+  void* d;
+  napi_value arr = new_typed(env, napi_float32_array, n, &d);
+  float fs = 0.0f;
+  AUSM_CALL(env, ausm_derived_field(c, (int)num(env, a[1]), (float*)d, &fs));
+  napi_value obj;
+  NAPI_CALL(env, napi_create_object(env, &obj));
+  napi_set_named_property(env, obj, "field", arr);
+  set_num(env, obj, "freestream", fs);
+  return obj;
+}
+
 static napi_value js_destroy(napi_env env, napi_callback_info info) {
   (void)env;
   (void)info;
@@ -379,6 +495,14 @@ static napi_value init(napi_env env, napi_value exports) {
       {"boundaryNaca4", NULL, js_naca4, NULL, NULL, NULL, napi_default, NULL},
       {"boundaryCircle", NULL, js_circle, NULL, NULL, NULL, napi_default, NULL},
       {"boundaryRect", NULL, js_rect, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryCst", NULL, js_cst, NULL, NULL, NULL, napi_default, NULL},
+      {"boundarySearsHaack", NULL, js_sears_haack, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryHaackProjectile", NULL, js_haack, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryPolygon", NULL, js_polygon, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryDat", NULL, js_dat, NULL, NULL, NULL, napi_default, NULL},
+      {"setPrecision", NULL, js_set_precision, NULL, NULL, NULL, napi_default, NULL},
+      {"setDtMode", NULL, js_set_dt_mode, NULL, NULL, NULL, napi_default, NULL},
+      {"derivedField", NULL, js_derived_field, NULL, NULL, NULL, napi_default, NULL},
       {"gridFromBoundary", NULL, js_grid_from_boundary, NULL, NULL, NULL, napi_default, NULL},
       {"setVertices", NULL, js_set_vertices, NULL, NULL, NULL, napi_default, NULL},
       {"getVertices", NULL, js_get_vertices, NULL, NULL, NULL, napi_default, NULL},

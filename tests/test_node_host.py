@@ -77,3 +77,56 @@ def test_node_frame_loop_matches_python_host(tmp_path):
     assert f["cl"] == js["forces"]["cl"] and f["cd"] == js["forces"]["cd"]
     assert js["actualInflowVel"] == act
     s.close()
+
+
+@needs_node
+def test_node_bodies_and_controls_match_library():
+    """The JS object menu (src/grid.js:258-263) and the new generator
+    bindings produce the library's points; pause and the adaptive
+    steps-per-frame controller (main.js:720-731, util.js:192-201) behave as
+    the reference's frame loop does, without a GPU (no context needed)."""
+    import ausm
+
+    dat = os.path.join(ROOT, "tests", "golden", "sc20012.dat")
+    js = r"""
+const m = require(process.argv[1]);
+const fs = require('fs');
+const M = 128;
+const text = fs.readFileSync(process.argv[2], 'utf8');
+const out = {
+  polygon: Array.from(m.objectBoundary('polygon', M)),
+  circle: Array.from(m.objectBoundary('circle', M)),
+  sears: Array.from(m.objectBoundary('sears-haack', M)),
+  haack: Array.from(m.objectBoundary('haack-projectile', M)),
+  dat: Array.from(m.objectBoundary('airfoil-dat', M, text)),
+  cst: Array.from(m.cstBoundary(M, [0.15, 0.15], [-0.1, -0.1])),
+};
+// controller arithmetic without a context: a solver-like object
+const S = Object.create(m.AusmSolver.prototype);
+Object.assign(S, {autoStepsPerFrame: true, targetFrameTime: 16, _lastTotal: 5, _lastFrameStart: 0,
+                  stepsPerFrame: 70, maxdt: 1e-4, _oldDt: null});
+S._adapt(20);            // interval 20 ms: overshoot 4 ms, compute 5 ms < 14 ms
+const a1 = S.stepsPerFrame;
+S.togglePause();
+const paused = !S.running;
+S._adapt(40);            // paused: unchanged
+const a2 = S.stepsPerFrame;
+S.togglePause();
+out.ctrl = [a1, paused, a2, S.running, S.maxdt];
+console.log(JSON.stringify(out));
+"""
+    r = subprocess.run([NODE, "-e", js, os.path.join(ROOT, "ausm-webgpu_amd", "node", "ausm.js"), dat],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    o = json.loads(r.stdout.strip().splitlines()[-1])
+    M = 128
+    get = lambda k: np.array(o[k]).reshape(M, 2)
+    assert np.array_equal(get("polygon"), ausm.polygon_boundary(M, 0.1, 4, 0.1, 0.0, -0.7))
+    assert np.array_equal(get("circle"), ausm.polygon_boundary(M, 0.1, 400, 1.0, 0.0, -0.7))
+    assert np.array_equal(get("sears"), ausm.sears_haack_boundary(M, 0.0002, 0.4, -0.7))
+    assert np.array_equal(get("haack"), ausm.haack_projectile_boundary(M))
+    with open(dat) as f:
+        assert np.array_equal(get("dat"), ausm.dat_boundary(M, f.read(), 0.3, -0.7)[0])
+    assert np.array_equal(get("cst"), ausm.cst_boundary(M, [0.15, 0.15], [-0.1, -0.1]))
+    # 70 + 1 (under budget) - round(4 * 0.05) = 71; paused: unchanged; resumed
+    assert o["ctrl"] == [71, True, 71, True, 1e-4]
