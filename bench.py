@@ -268,6 +268,24 @@ def serving_leg(serve, n, cap, drop, rates, grid_iters, dt_mode, precision, n_ch
                 f"requests: max |dCl| {acc['max_abs_cl_err_same_criterion']} at the same "
                 f"criterion, {acc['max_abs_cl_err']} against the converged FP64 answer")
         out["variants"].append(var)
+    # the engine where single solves leave the GPU idle: 64 small requests
+    # (128 x 64 cells, first order, all converge)
+    small = serve.make_requests(64, seed=7, sizes=[(128, 64)], mach_range=(0.1, 10.0),
+                                log_mach=True, aoa_range=(-5.0, 10.0), max_steps=cap, res_drop=drop)
+    for r in small:
+        r["limiter"] = 3
+    serve.make_grids(small, iters=grid_iters)
+    sb = serve.run_batched(small, None, check_every=100, max_solves=64, dt_mode=dt_mode,
+                           precision=precision)
+    ss = serve.run_sequential(small, None, check_every=100, dt_mode=dt_mode, precision=precision)
+    out["small_grid_mix"] = {
+        "workload": "64 requests of 128x64 cells, first order, drawn as above",
+        "batched": strip(sb), "one_by_one": strip(ss),
+        "speedup_converged_solves_per_s": (sb["converged_solves_per_s"] / ss["converged_solves_per_s"]
+                                           if ss["converged_solves_per_s"] > 0 else None)}
+    log(f"[bench] serving small-grid mix: {sb['converged_solves_per_s']:.1f} vs "
+        f"{ss['converged_solves_per_s']:.1f} converged solves/s one by one, p99 "
+        f"{sb['p99_latency_s']:.2f} s vs {ss['p99_latency_s']:.2f} s")
     return out
 
 
