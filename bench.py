@@ -37,6 +37,7 @@ sys.path.insert(0, os.path.join(ROOT, "ausm-webgpu_amd"))
 ALG_BYTES_PER_CELL_STEP = 152.0  # 8 x 16 B state streams + 3 x 8 B vertex
 HBM_PEAK_GBPS = 8000.0
 METRIC = "cell-steps/s & % HBM roofline; batched converged solves/s + p50/p99 latency"
+TILE_ROWS = 2
 
 
 def log(msg):
@@ -84,6 +85,10 @@ def build_workload(ausm, M, N, grid_iters):
     s.set_flow(2.0, 0.0)
     s.set_scheme(ausm.AUSMUP, ausm.VANALBADA, 0.0, 0.75)
     s.set_cfl(0.5, 1.0, 1)
+    # two rows per wavefront tile: the fastest shape at this size in a sweep
+    # over 1 / 2 / 4 / 8 (28.7 vs 31.0 / 37.3 / 63.4 us per step), fixed so
+    # that timing, trace and counter runs all measure the same kernel
+    s.set_tile(TILE_ROWS)
     return s, t_grid
 
 
@@ -396,6 +401,7 @@ def main():
                                "AoA 0, AUSM+-up + MUSCL van Albada, TVD-RK3, CFL 0.5, "
                                "free-stream start",
                    "M": M, "N": N, "math": args.math, "dt_refresh_every": 1,
+                   "tile_rows_per_wavefront": TILE_ROWS,
                    "parallelism": "single GPU"},
         "pct_hbm_roofline_whole_step": 100.0 * whole_gbps / HBM_PEAK_GBPS,
         "steps_per_s": args.steps / (ms / 1e3),

@@ -230,9 +230,10 @@ int ausm_eval_residual64(ausm_ctx* ctx, double* res);
 
 /* Timing hook, no counterpart in the reference (it times whole passes with
  * timestamp queries, main.js:597-602): launches each of the three stage
- * kernels `reps` times on the context's stream, each launch bracketed by HIP
- * events, and returns the mean duration of each stage in ms.  The solution
- * is left untouched (stage outputs go to the scratch buffers). */
+ * kernels `reps` times back to back on the context's stream between one
+ * pair of HIP events and returns the mean duration per launch of each stage
+ * in ms.  The solution is left untouched (stage outputs go to the scratch
+ * buffers). */
 int ausm_profile_stages(ausm_ctx* ctx, int reps, float* ms3);
 
 /* ---- batched solves and continuous-batching service ---------------------
