@@ -159,7 +159,15 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
 
   R ws_acc = R(0);
 
-  if (j0 < N) {
+  // Waves whose rows start past the grid still load (the rows are clamped,
+  // so every access is in bounds) and take part in the workgroup barriers,
+  // but compute and store nothing.
+  const bool active = j0 < N;
+  const bool full = j0 + TY <= N;  // all TY rows inside the grid
+  // vertical face between this workgroup's waves: wave w's top face is wave
+  // w+1's bottom face, computed once and passed through LDS
+  __shared__ R s_face[kWavesPerBlock][5][kWave];
+  {
     const R* __restrict__ qin = a.qin;
     // ---- prefetch: every row the tile reads, issued before any arithmetic.
     // Stage-input rows j0-2 .. j0+TY+1 (clamped to the interior; the clamped
@@ -208,13 +216,13 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     };
     auto row_of = [&](int j) -> Cons { return ld_cons(row_ptr(qin, j), PS, bc); };
     Prim gW{}, gF1{}, gF2{};
-    if (j0 < 2) {
+    if (active && j0 < 2) {
       R nx, ny, len;
       normal_exact(ld_row(a.vx, bc) - ld_row(a.vx, bcr), ld_row(a.vy, bc) - ld_row(a.vy, bcr), nx, ny,
                    len);
       gW = to_prim(wall_ghost(row_of(0), nx, ny));
     }
-    if (j0 + TY + 2 > N) {
+    if (active && j0 + TY + 2 > N) {
       const R* vxN = row_ptr(a.vx, N);
       const R* vyN = row_ptr(a.vy, N);
       R nx, ny, len;
@@ -233,6 +241,26 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       return to_prim(raw[k]);
     };
 
+    // top face of a full tile (face index j0 + TY, window slots TY..TY+3),
+    // first, so that the wave above can start from it
+    Flux Gtop{};
+    R lTop = R(0);
+    if (full) {
+      R nx, ny;
+      const R vt = vxr[TY], wt = vyr[TY];
+      face_normal<FAST>(vt - from_right(vt), wt - from_right(wt), nx, ny, lTop);
+      const Flux G = face_flux<SCHEME, FAST>(P, lim, prim_of(TY), prim_of(TY + 1), prim_of(TY + 2),
+                                             prim_of(TY + 3), nx, ny);
+      Gtop = Flux{G.f0 * lTop, G.f1 * lTop, G.f2 * lTop, G.f3 * lTop};
+      s_face[wid][0][lane] = Gtop.f0;
+      s_face[wid][1][lane] = Gtop.f1;
+      s_face[wid][2][lane] = Gtop.f2;
+      s_face[wid][3][lane] = Gtop.f3;
+      s_face[wid][4][lane] = lTop;
+    }
+    __syncthreads();
+
+    if (active) {
     const Prim w_m2 = prim_of(0);
     Prim w_m1 = prim_of(1);
     Prim w_0 = prim_of(2);
@@ -240,10 +268,15 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     R vx0 = vxr[0], vy0 = vyr[0];
     R vx0r = from_right(vx0), vy0r = from_right(vy0);
 
-    // bottom face of row j0: vertical flux at face index j0 (ausm.js:191-205)
+    // bottom face of row j0: vertical flux at face index j0 (ausm.js:191-205);
+    // above the first wave it is the wave below's top face (its tile is full)
     Flux Gd;
     R lD;
-    {
+    if (wid > 0) {
+      Gd = Flux{s_face[wid - 1][0][lane], s_face[wid - 1][1][lane], s_face[wid - 1][2][lane],
+                s_face[wid - 1][3][lane]};
+      lD = s_face[wid - 1][4][lane];
+    } else {
       R nx, ny;
       face_normal<FAST>(vx0 - vx0r, vy0 - vy0r, nx, ny, lD);
       const Flux G = face_flux<SCHEME, FAST>(P, lim, w_m2, w_m1, w_0, w_p1, nx, ny);
@@ -267,11 +300,19 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       const Flux FX = face_flux<SCHEME, FAST>(P, lim, wl2, wl, w_0, wr, nxX, nyX);
       const Flux PX{FX.f0 * lL, FX.f1 * lL, FX.f2 * lL, FX.f3 * lL};
 
-      // vertical face at the top of cell (i, r): stencil r-1..r+2
-      R nxY, nyY, lU;
-      face_normal<FAST>(vx1 - vx1r, vy1 - vy1r, nxY, nyY, lU);
-      const Flux FY = face_flux<SCHEME, FAST>(P, lim, w_m1, w_0, w_p1, w_p2, nxY, nyY);
-      const Flux Gu{FY.f0 * lU, FY.f1 * lU, FY.f2 * lU, FY.f3 * lU};
+      // vertical face at the top of cell (i, r): stencil r-1..r+2; the last
+      // row of a tile (reached only when the tile is full) has it already
+      R lU;
+      Flux Gu;
+      if (t == TY - 1) {
+        Gu = Gtop;
+        lU = lTop;
+      } else {
+        R nxY, nyY;
+        face_normal<FAST>(vx1 - vx1r, vy1 - vy1r, nxY, nyY, lU);
+        const Flux FY = face_flux<SCHEME, FAST>(P, lim, w_m1, w_0, w_p1, w_p2, nxY, nyY);
+        Gu = Flux{FY.f0 * lU, FY.f1 * lU, FY.f2 * lU, FY.f3 * lU};
+      }
 
       // right face of cell (i, r) is the left face of cell (i+1, r)
       const Flux PR{from_right(PX.f0), from_right(PX.f1), from_right(PX.f2), from_right(PX.f3)};
@@ -397,8 +438,10 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       Gd = Gu;
       lD = lU;
     }
+    }  // active
   }
-
+  // the LDS faces may be rewritten by the next tile of a persistent loop
+  __syncthreads();
   return ws_acc;
 }
 
