@@ -30,8 +30,9 @@ __global__ __launch_bounds__(256) void stage_kernel(const StageArgs a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const float dt = stage_dt(P, a, blockIdx.x == 0);
-  const float ws_acc = stage_tile<SCHEME, FAST, TY>(P, a, tx, tyb, dt);
+  const DtAhead<float> ahead = stage_dt_load(a);
+  const float ws_acc = stage_tile<SCHEME, FAST, TY>(
+      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
@@ -43,8 +44,9 @@ __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const double dt = stage_dt(P, a, blockIdx.x == 0);
-  const double ws_acc = stage_tile<SCHEME, false, TY, double>(P, a, tx, tyb, dt);
+  const DtAhead<double> ahead = stage_dt_load(a);
+  const double ws_acc = stage_tile<SCHEME, false, TY, double>(
+      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
@@ -55,8 +57,9 @@ __global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const float dt = stage_dt(P, a, blockIdx.x == 0);
-  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true>(P, a, tx, tyb, dt);
+  const DtAhead<float> ahead = stage_dt_load(a);
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true>(
+      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 

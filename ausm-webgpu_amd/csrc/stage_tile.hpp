@@ -1,6 +1,8 @@
 // The fused RK-stage tile body shared by the single-grid kernel and the
 // batched persistent kernel (see stage_kernels.hip for the design).
 #pragma once
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace ausm {
@@ -133,9 +135,12 @@ __device__ __forceinline__ void st_cons(R* row, size_t PS, uint32_t boff, R a, R
   st_row(row + 3 * PS, boff, d);
 }
 
-template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false>
+// dt_arg is the stage's dt, or a callable that returns it; a callable is
+// evaluated once, after the workgroup barrier, so that a reduction behind it
+// can overlap the tile's loads and first face.
+template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false, typename DtF = R>
 __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
-                                        int tyb, R dt) {
+                                        int tyb, DtF dt_arg) {
   static_assert(!FAST || sizeof(R) == 4, "the fast arithmetic is FP32 only");
   using Prim = PrimT<R>;
   using Cons = ConsT<R>;
@@ -259,6 +264,9 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       s_face[wid][4][lane] = lTop;
     }
     __syncthreads();
+    R dt;
+    if constexpr (std::is_invocable_v<DtF>) dt = dt_arg();
+    else dt = dt_arg;
 
     if (active) {
     const Prim w_m2 = prim_of(0);
@@ -446,14 +454,39 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
 }
 
 // dt = min(maxdt, cfl / max wave speed) (ausm.js:457).  On a fresh stage 1
-// every wavefront reduces the per-workgroup maxima of the last refresh.
+// every wavefront reduces the per-workgroup maxima of the last refresh: the
+// first kDtAhead per lane are loaded by stage_dt_load at kernel entry, in
+// one batch that overlaps the tile's own loads, and reduced by
+// stage_dt_finish where dt is first needed (the same order of maxima as a
+// plain lane-strided loop, so dt does not depend on the split).
+constexpr int kDtAhead = 8;
 template <typename R>
-__device__ __forceinline__ R stage_dt(const DevParamsT<R>& P, const StageArgsT<R>& a, bool writer) {
+struct DtAhead {
+  R v[kDtAhead];
+};
+template <typename R>
+__device__ __forceinline__ DtAhead<R> stage_dt_load(const StageArgsT<R>& a) {
+  DtAhead<R> d;
+  const int lane = threadIdx.x & 63;
+  if (a.stage == 1 && a.fresh) {
+#pragma unroll
+    for (int k = 0; k < kDtAhead; ++k) {
+      const int i = lane + k * kWave;
+      d.v[k] = i < a.n_ws_in ? a.ws_in[i] : R(0);
+    }
+  }
+  return d;
+}
+template <typename R>
+__device__ __forceinline__ R stage_dt_finish(const DevParamsT<R>& P, const StageArgsT<R>& a,
+                                             bool writer, const DtAhead<R>& d) {
   const int lane = threadIdx.x & 63;
   R maxws;
   if (a.stage == 1 && a.fresh) {
     R m = R(0);
-    for (int i = lane; i < a.n_ws_in; i += kWave) m = nan_max(m, a.ws_in[i]);
+#pragma unroll
+    for (int k = 0; k < kDtAhead; ++k) m = nan_max(m, d.v[k]);
+    for (int i = lane + kDtAhead * kWave; i < a.n_ws_in; i += kWave) m = nan_max(m, a.ws_in[i]);
     maxws = wave_max(m);
     if (writer && threadIdx.x == 0) *a.maxws_cur = maxws;
   } else {
