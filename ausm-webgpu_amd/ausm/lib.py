@@ -74,6 +74,7 @@ SIGNATURES = {
     "ausm_sync": (_i, [_vp]),
     "ausm_boundary_naca4": (_i, [_d, _d, _d, _d, _d, _d, _i, _i, _vp]),
     "ausm_boundary_cst": (_i, [_vp, _vp, _i, _d, _d, _d, _d, _i, _i, _vp]),
+    "ausm_boundary_bezier": (_i, [_vp, _i, _vp, _i, _d, _d, _d, _i, _i, _vp]),
     "ausm_boundary_circle": (_i, [_d, _i, _vp]),
     "ausm_boundary_sears_haack": (_i, [_d, _d, _d, _i, _vp]),
     "ausm_boundary_haack_projectile": (_i, [_d, _d, _d, _d, _d, _d, _d, _i, _vp]),

@@ -53,6 +53,20 @@ def cst_boundary(M, a_upper, a_lower, te_gap=0.0, size=0.5, x_offset=0.0, expone
     return out
 
 
+def bezier_boundary(M, upper, lower, size=0.5, x_offset=0.0, exponent=1.4, spacing="power"):
+    """Bezier body of M points (no counterpart in the reference): upper and
+    lower are (n, 2) control polygons in chord units from the shared
+    leading-edge point (t = 0) to each trailing-edge point (t = 1); the curve
+    parameter takes naca4_boundary's chordwise stations and point order."""
+    cu = np.ascontiguousarray(np.asarray(upper, dtype=np.float64).reshape(-1, 2))
+    cl = np.ascontiguousarray(np.asarray(lower, dtype=np.float64).reshape(-1, 2))
+    out = np.zeros((M, 2), dtype=np.float64)
+    sp = L.SPACING_COSINE if spacing == "cosine" else L.SPACING_POWER
+    check(L.load().ausm_boundary_bezier(ptr(cu), cu.shape[0], ptr(cl), cl.shape[0], size, x_offset,
+                                        exponent, sp, M, ptr(out)))
+    return out
+
+
 def sears_haack_boundary(M, V=0.0002, L=0.4, x_offset=-0.7):
     """Sears-Haack body of volume V and length L (src/grid.js:53-61; defaults
     are the reference's object menu, src/grid.js:262)."""

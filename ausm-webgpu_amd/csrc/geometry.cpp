@@ -1,6 +1,6 @@
 // Body and far-field generators of libausm (host code, FP64): the
 // reference's host-side boundary generators (src/grid.js, src/airfoilDatParser.js)
-// and the Kulfan CST section.  Each fills M points (x, y), counter-clockwise,
+// and the Kulfan CST and Bezier sections.  Each fills M points (x, y), counter-clockwise,
 // in the point order the O-grid's body row (j = 0) or far-field row (j = N)
 // expects.
 #include <cctype>
@@ -91,6 +91,56 @@ int ausm_boundary_cst(const double* a_upper, const double* a_lower, int n_coef, 
     const double y = upper ? cst_surface(a_upper, n_coef, xc) + xc * (0.5 * te_gap)
                            : cst_surface(a_lower, n_coef, xc) - xc * (0.5 * te_gap);
     xy[2 * i] = (xc - 0.5) * chord + x_offset;
+    xy[2 * i + 1] = y * chord;
+  }
+  return AUSM_OK;
+}
+
+// Bezier section: upper and lower surfaces are Bezier curves of n_upper and
+// n_lower control points (x, y) in chord units, each from the leading edge
+// (t = 0) to the trailing edge (t = 1).  The curve parameter is spaced as the
+// chordwise stations of the NACA generator (point order CCW from the upper
+// trailing edge); points are evaluated by de Casteljau's construction.
+static void bezier_point(const double* cp, int n, double t, double* x, double* y) {
+  double bx[32], by[32];
+  for (int k = 0; k < n; ++k) {
+    bx[k] = cp[2 * k];
+    by[k] = cp[2 * k + 1];
+  }
+  for (int r = n - 1; r > 0; --r)
+    for (int k = 0; k < r; ++k) {
+      bx[k] = (1.0 - t) * bx[k] + t * bx[k + 1];
+      by[k] = (1.0 - t) * by[k] + t * by[k + 1];
+    }
+  *x = bx[0];
+  *y = by[0];
+}
+
+int ausm_boundary_bezier(const double* upper, int n_upper, const double* lower, int n_lower,
+                         double size, double x_offset, double exponent, int spacing, int M,
+                         double* xy) {
+  if (!xy || !upper || !lower || M < 4) return set_error(AUSM_ERR_ARG, "bad Bezier arguments");
+  if (n_upper < 2 || n_upper > 32 || n_lower < 2 || n_lower > 32)
+    return set_error(AUSM_ERR_ARG, "a Bezier surface needs 2..32 control points");
+  for (int k = 0; k < 2 * n_upper; ++k)
+    if (!std::isfinite(upper[k])) return set_error(AUSM_ERR_ARG, "non-finite Bezier control point");
+  for (int k = 0; k < 2 * n_lower; ++k)
+    if (!std::isfinite(lower[k])) return set_error(AUSM_ERR_ARG, "non-finite Bezier control point");
+  if (upper[0] != lower[0] || upper[1] != lower[1])
+    return set_error(AUSM_ERR_ARG, "the two surfaces must start at one leading-edge point");
+  const double chord = 2.0 * size;
+  for (int i = 0; i < M; ++i) {
+    const double s = (double)i / ((double)M / 2.0);
+    const bool up = s <= 1.0;
+    double tc;
+    if (spacing == AUSM_SPACING_COSINE)
+      tc = up ? 0.5 * (1.0 + std::cos(M_PI * s)) : 0.5 * (1.0 - std::cos(M_PI * (s - 1.0)));
+    else
+      tc = up ? std::pow(1.0 - s, exponent) : std::pow(s - 1.0, exponent);
+    double x, y;
+    if (up) bezier_point(upper, n_upper, tc, &x, &y);
+    else bezier_point(lower, n_lower, tc, &x, &y);
+    xy[2 * i] = (x - 0.5) * chord + x_offset;
     xy[2 * i + 1] = y * chord;
   }
   return AUSM_OK;

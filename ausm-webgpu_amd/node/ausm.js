@@ -46,6 +46,11 @@ const cstBoundary = (M, aUpper, aLower, teGap = 0, size = 0.5, xOffset = 0, expo
   native.boundaryCst(Float64Array.from(aUpper), Float64Array.from(aLower), teGap, size, xOffset, exponent,
                      spacing, M);
 
+// upper / lower: control polygons [[x, y], ...] from the shared leading edge
+const bezierBoundary = (M, upper, lower, size = 0.5, xOffset = 0, exponent = 7 / 5, spacing = 0) =>
+  native.boundaryBezier(Float64Array.from(upper.flat()), Float64Array.from(lower.flat()), size, xOffset,
+                        exponent, spacing, M);
+
 // the object menu of the reference with its settings (src/grid.js:258-263)
 function objectBoundary(type, M, datText) {
   switch (type) {
@@ -217,5 +222,5 @@ class AusmSolver {
 module.exports = {
   AusmSolver, solvers, limiters, fields, precisions, naca4Boundary, circularOuterBoundary,
   rectangularOuterBoundary, polygonBoundary, searsHaackBoundary, haackProjectileBoundary, datBoundary,
-  cstBoundary, objectBoundary, INIT_REST, INIT_FREESTREAM, native,
+  cstBoundary, bezierBoundary, objectBoundary, INIT_REST, INIT_FREESTREAM, native,
 };

@@ -163,6 +163,25 @@ static napi_value js_cst(napi_env env, napi_callback_info info) {
   return out;
 }
 
+static napi_value js_bezier(napi_env env, napi_callback_info info) {
+  napi_value a[7];
+  if (!get_args(env, info, 7, a)) return NULL;
+  size_t nu = 0, nl = 0;
+  const double* cu = (const double*)typed(env, a[0], napi_float64_array, &nu);
+  if (!cu) return NULL;
+  const double* cl = (const double*)typed(env, a[1], napi_float64_array, &nl);
+  if (!cl) return NULL;
+  if (nu % 2 || nl % 2) return throw_ausm(env, AUSM_ERR_ARG);
+  const int M = (int)num(env, a[6]);
+  if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
+  void* d;
+  napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  AUSM_CALL(env, ausm_boundary_bezier(cu, (int)(nu / 2), cl, (int)(nl / 2), num(env, a[2]),
+                                      num(env, a[3]), num(env, a[4]), (int)num(env, a[5]), M,
+                                      (double*)d));
+  return out;
+}
+
 static napi_value js_sears_haack(napi_env env, napi_callback_info info) {
   napi_value a[4];
   if (!get_args(env, info, 4, a)) return NULL;
@@ -496,6 +515,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"boundaryCircle", NULL, js_circle, NULL, NULL, NULL, napi_default, NULL},
       {"boundaryRect", NULL, js_rect, NULL, NULL, NULL, napi_default, NULL},
       {"boundaryCst", NULL, js_cst, NULL, NULL, NULL, napi_default, NULL},
+      {"boundaryBezier", NULL, js_bezier, NULL, NULL, NULL, napi_default, NULL},
       {"boundarySearsHaack", NULL, js_sears_haack, NULL, NULL, NULL, napi_default, NULL},
       {"boundaryHaackProjectile", NULL, js_haack, NULL, NULL, NULL, napi_default, NULL},
       {"boundaryPolygon", NULL, js_polygon, NULL, NULL, NULL, napi_default, NULL},

@@ -108,6 +108,16 @@ int ausm_boundary_naca4(double m, double p, double t, double size, double x_offs
 int ausm_boundary_cst(const double* a_upper, const double* a_lower, int n_coef, double te_gap,
                       double size, double x_offset, double exponent, int spacing, int M,
                       double* xy);
+/* Bezier section; no counterpart in the reference.  upper and lower hold
+ * n_upper and n_lower (2..32) control points (x, y) in chord units, each
+ * surface from the leading edge (t = 0, one point shared by both) to its
+ * trailing-edge point (t = 1); the curve parameter takes the chordwise
+ * stations of ausm_boundary_naca4 (spacing, exponent), and the chord 2*size
+ * is placed as there.  Non-finite control points, or surfaces that do not
+ * share the leading-edge point, are AUSM_ERR_ARG. */
+int ausm_boundary_bezier(const double* upper, int n_upper, const double* lower, int n_lower,
+                         double size, double x_offset, double exponent, int spacing, int M,
+                         double* xy);
 /* Bodies the reference's object menu offers (src/grid.js:258-263), in the
  * point order of ausm_boundary_naca4:
  *   Sears-Haack body of volume V and length L (generateSearsHaackBoundary,

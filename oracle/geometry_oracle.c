@@ -76,6 +76,43 @@ void geo_cst_point(const double* au, const double* al, int ncoef, double te_gap,
   xy[1] = y * chord;
 }
 
+/* Bezier section (no counterpart in the reference): each surface is
+ * evaluated in Bernstein form, sum_k C(n-1,k) t^k (1-t)^(n-1-k) P_k, with the
+ * curve parameter t at the chordwise stations of geo_naca4_point; the upper
+ * control polygon serves s <= 1, the lower one s > 1.  Pinned by the
+ * mathematics (endpoint interpolation, linear precision, the parabola of a
+ * quadratic curve, degree elevation, mirror symmetry) in
+ * tests/test_oracle_pins.py. */
+static void geo_bernstein(const double* cp, int n, double t, double* x, double* y) {
+  const int d = n - 1;
+  double fact[33];
+  fact[0] = 1.0;
+  for (int k = 1; k <= d; ++k) fact[k] = fact[k - 1] * (double)k;
+  double sx = 0.0, sy = 0.0;
+  for (int k = 0; k <= d; ++k) {
+    const double b = (fact[d] / (fact[k] * fact[d - k])) * pow(t, (double)k) * pow(1.0 - t, (double)(d - k));
+    sx += b * cp[2 * k];
+    sy += b * cp[2 * k + 1];
+  }
+  *x = sx;
+  *y = sy;
+}
+
+void geo_bezier_point(const double* cu, int nu, const double* cl, int nl, double size,
+                      double x_offset, double expo, int spacing, int M, int i, double* xy) {
+  const double s = (double)i / ((double)M / 2.0);
+  const double chord = 2.0 * size;
+  const int upper = s <= 1.0;
+  double tc;
+  if (spacing == 0) tc = upper ? pow(1.0 - s, expo) : pow(s - 1.0, expo);
+  else tc = upper ? 0.5 * (1.0 + cos(M_PI * s)) : 0.5 * (1.0 - cos(M_PI * (s - 1.0)));
+  double x, y;
+  if (upper) geo_bernstein(cu, nu, tc, &x, &y);
+  else geo_bernstein(cl, nl, tc, &x, &y);
+  xy[0] = (x - 0.5) * chord + x_offset;
+  xy[1] = y * chord;
+}
+
 /* generateCircularOuterBoundary (src/grid.js:232-237) */
 void geo_circle_point(double radius, int M, int i, double* xy) {
   const double t = (double)i / (double)M;

@@ -98,6 +98,7 @@ def _declare(L, prefixes=(("orc32_", ctypes.c_float), ("orc64_", ctypes.c_double
         getattr(L, n).argtypes = [cd, cd, cd, cd, cd, cd, ci, ci, ci, vp]
     L.geo_circle_point.argtypes = [cd, ci, ci, vp]
     L.geo_cst_point.argtypes = [vp, vp, ci, cd, cd, cd, cd, ci, ci, ci, vp]
+    L.geo_bezier_point.argtypes = [vp, ci, vp, ci, cd, cd, cd, ci, ci, ci, vp]
     L.geo_sears_haack_point.argtypes = [cd, cd, cd, ci, ci, vp]
     L.geo_haack_point.argtypes = [cd, cd, cd, cd, cd, cd, cd, ci, ci, vp]
     L.geo_polygon_point.argtypes = [cd, ci, cd, cd, cd, ci, ci, vp]
@@ -166,6 +167,16 @@ def cst_boundary(M, a_upper, a_lower, te_gap=0.0, size=0.5, x_offset=0.0, expo=1
     for i in range(M):
         lib().geo_cst_point(_ptr(au), _ptr(al), au.size, te_gap, size, x_offset, expo, spacing, M,
                             i, _ptr(out[i]))
+    return out
+
+
+def bezier_boundary(M, upper, lower, size=0.5, x_offset=0.0, expo=1.4, spacing=1):
+    cu = np.ascontiguousarray(np.asarray(upper, dtype=np.float64).reshape(-1, 2))
+    cl = np.ascontiguousarray(np.asarray(lower, dtype=np.float64).reshape(-1, 2))
+    out = np.zeros((M, 2), dtype=np.float64)
+    for i in range(M):
+        lib().geo_bezier_point(_ptr(cu), cu.shape[0], _ptr(cl), cl.shape[0], size, x_offset, expo,
+                               spacing, M, i, _ptr(out[i]))
     return out
 
 

@@ -61,6 +61,28 @@ def test_naca_generator_matches_oracle(code, spacing):
     assert np.allclose(a, b, rtol=0, atol=1e-15)
 
 
+@pytest.mark.parametrize("spacing", ["power", "cosine"])
+def test_bezier_generator_matches_oracle(spacing):
+    """The library's de Casteljau evaluation against the oracle's Bernstein
+    form, and its argument checks."""
+    import ausm
+    from ausm import bezier_boundary
+
+    from oracle_lib import bezier_boundary as o_bez
+
+    cu = [[0.0, 0.0], [0.0, 0.06], [0.25, 0.1], [0.6, 0.08], [0.85, 0.03], [1.0, 0.0]]
+    cl = [[0.0, 0.0], [0.0, -0.05], [0.3, -0.06], [0.7, -0.01], [1.0, 0.0]]
+    a = bezier_boundary(512, cu, cl, size=0.5, x_offset=0.1, spacing=spacing)
+    b = o_bez(512, cu, cl, size=0.5, x_offset=0.1, spacing=1 if spacing == "cosine" else 0)
+    assert np.allclose(a, b, rtol=0, atol=1e-15)
+    with pytest.raises(ausm.AusmError, match="leading-edge"):
+        bezier_boundary(64, cu, [[0.0, 0.001], [1.0, 0.0]])
+    with pytest.raises(ausm.AusmError, match="2..32"):
+        bezier_boundary(64, cu, [[0.0, 0.0]])
+    with pytest.raises(ausm.AusmError, match="non-finite"):
+        bezier_boundary(64, cu, [[0.0, 0.0], [float("nan"), 0.0]])
+
+
 def test_far_field_generators_match_oracle():
     from ausm import circle_boundary, rect_boundary
 

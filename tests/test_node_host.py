@@ -100,6 +100,7 @@ const out = {
   haack: Array.from(m.objectBoundary('haack-projectile', M)),
   dat: Array.from(m.objectBoundary('airfoil-dat', M, text)),
   cst: Array.from(m.cstBoundary(M, [0.15, 0.15], [-0.1, -0.1])),
+  bez: Array.from(m.bezierBoundary(M, [[0, 0], [0, 0.05], [0.5, 0.08], [1, 0]], [[0, 0], [0, -0.03], [1, 0]])),
 };
 // controller arithmetic without a context: a solver-like object
 const S = Object.create(m.AusmSolver.prototype);
@@ -128,5 +129,7 @@ console.log(JSON.stringify(out));
     with open(dat) as f:
         assert np.array_equal(get("dat"), ausm.dat_boundary(M, f.read(), 0.3, -0.7)[0])
     assert np.array_equal(get("cst"), ausm.cst_boundary(M, [0.15, 0.15], [-0.1, -0.1]))
+    assert np.array_equal(get("bez"), ausm.bezier_boundary(
+        M, [[0, 0], [0, 0.05], [0.5, 0.08], [1, 0]], [[0, 0], [0, -0.03], [1, 0]]))
     # 70 + 1 (under budget) - round(4 * 0.05) = 71; paused: unchanged; resumed
     assert o["ctrl"] == [71, True, 71, True, 1e-4]

@@ -382,6 +382,53 @@ def test_cst_closed_forms():
     assert math.isclose(bt[0, 1], 0.002, abs_tol=1e-16) and bt[0, 0] == 0.5
 
 
+def test_bezier_closed_forms():
+    """Bezier oracle (Bernstein form): endpoints interpolate the end control
+    points; control points evenly spaced on a line give that line with
+    x = t (linear precision); the quadratic (0,0), (1/2,h), (1,0) is the
+    parabola y = 2h x(1-x); a degree-elevated polygon draws the same curve;
+    a lower polygon mirrored from the upper mirrors the section."""
+    from oracle_lib import bezier_boundary as o_bez
+
+    M = 512
+    up = np.arange(M) <= M // 2
+    # linear precision on both sides: y = +-0.1 x, stations x = t
+    n = 5
+    line_u = np.array([[i / (n - 1), 0.1 * i / (n - 1)] for i in range(n)])
+    line_l = line_u * [1.0, -1.0]
+    b = o_bez(M, line_u, line_l, size=0.5, x_offset=0.0, expo=1.4, spacing=0)
+    s = np.arange(M) / (M / 2)
+    tc = np.where(up, (1 - np.minimum(s, 1)) ** 1.4, (np.maximum(s, 1) - 1) ** 1.4)
+    assert np.allclose(b[:, 0] + 0.5, tc, rtol=0, atol=2e-16)
+    assert np.allclose(b[:, 1], np.where(up, 0.1, -0.1) * tc, rtol=0, atol=2e-16)
+    # endpoints: index 0 is the upper trailing edge, M/2 the leading edge
+    cu = np.array([[0.0, 0.0], [0.0, 0.05], [0.3, 0.09], [0.7, 0.05], [1.0, 0.002]])
+    cl = np.array([[0.0, 0.0], [0.0, -0.04], [0.4, -0.05], [1.0, -0.002]])
+    bb = o_bez(M, cu, cl, size=0.5, spacing=1)
+    assert np.allclose(bb[0], cu[-1] - [0.5, 0.0], rtol=0, atol=1e-16)
+    assert np.allclose(bb[M // 2], cu[0] - [0.5, 0.0], rtol=0, atol=1e-16)
+    # quadratic = parabola y = 2 h x (1 - x), x = t
+    h = 0.12
+    q = np.array([[0.0, 0.0], [0.5, h], [1.0, 0.0]])
+    bq = o_bez(M, q, q * [1.0, -1.0], size=0.5, spacing=1)
+    x = bq[:, 0] + 0.5
+    assert np.allclose(bq[:, 1], np.where(up, 1.0, -1.0) * 2 * h * x * (1 - x), rtol=0, atol=1e-16)
+
+    # degree elevation: P'_i = i/(n+1) P_(i-1) + (1 - i/(n+1)) P_i
+    def elevate(p):
+        n1 = p.shape[0]
+        return np.array([(i / n1) * (p[i - 1] if i > 0 else 0.0) +
+                         (1 - i / n1) * (p[i] if i < n1 else 0.0) for i in range(n1 + 1)])
+
+    be = o_bez(M, elevate(cu), elevate(elevate(cl)), size=0.5, spacing=1)
+    assert np.allclose(bb, be, rtol=0, atol=1e-15)
+    # mirror
+    bm = o_bez(M, cu, cu * [1.0, -1.0], size=0.5, spacing=1)
+    # (stations i and M - i agree to an ulp of the cosine)
+    assert np.allclose(bm[1:M // 2, 1], -bm[M - 1:M // 2:-1, 1], rtol=0, atol=4e-16)
+    assert np.allclose(bm[1:M // 2, 0], bm[M - 1:M // 2:-1, 0], rtol=0, atol=4e-16)
+
+
 def test_ref32_tracks_ref64():
     """On a smooth subsonic case the FP32 and FP64 oracles agree to FP32
     rounding growth over a few dozen steps."""
