@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--math", default="fast", choices=["fast", "exact"])
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-literal", action="store_true",
+                    help="skip the 510x381 figure")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--batch-solves", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=5000)
@@ -365,6 +367,20 @@ def main():
     log(f"[bench] stage kernel {avg_launch_s * 1e6:.2f} us per launch in the timed region; "
         f"isolated stages {[round(x * 1e3, 2) for x in stage_ms]} us")
 
+    # the same workload at the reference README's literal 510 x 381 (the
+    # O-grid the reference's default grid settings give), for comparability
+    literal = None
+    if not args.no_literal:
+        ml, nl = 510, 381
+        sl, _ = build_workload(ausm, ml, nl, args.grid_iters)
+        sl.set_math(args.math)
+        nst = max(1, args.steps // 5)
+        ms_l, _, _ = time_steps(sl, ev, nst, min(args.warmup, 100))
+        literal = {"M": ml, "N": nl, "value": ml * nl * nst / (ms_l / 1e3), "unit": "cell-steps/s",
+                   "ms_per_step": ms_l / nst, "steps": nst}
+        log(f"[bench] {ml}x{nl}: {literal['value']:.3e} cell-steps/s")
+        sl.close()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline")
@@ -420,6 +436,7 @@ def main():
                      "launch_source": "HIP events over the timed region / (3 x steps)",
                      "isolated_stage_us": [x * 1e3 for x in stage_ms]},
         "cpu_baseline": cpu,
+        "literal_510x381": literal,
         "batched": batched,
         "serving": serving,
     }

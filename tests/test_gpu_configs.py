@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import ausm
-from oracle_lib import MINMOD, SLAU2, VANALBADA, Solver, make_params
+from oracle_lib import AUSMUP, MINMOD, SLAU2, VANALBADA, Solver, make_params
 
 pytestmark = pytest.mark.gpu
 
@@ -58,4 +58,40 @@ def test_naca0012_256x128_1000_steps(limiter):
     assert abs(f32["cl"] - f64["cl"]) < 1e-4
     assert abs(f32["cd"] - f64["cd"]) < 1e-4
     assert rel.max() < 1e-3
+    s.close()
+
+
+def test_literal_510x381_supersonic():
+    """The reference README's literal grid size, 510 x 381 (M not a multiple
+    of the 60-column wavefront strip, N odd, so partial strips and partial
+    row blocks are both exercised at full size): NACA 2412 in the
+    rectangular far field, 2000 Jacobi sweeps of the FP64 grid solve, Mach
+    2.0, AUSM+-up + van Albada, CFL 0.5, 40 steps from the free stream.
+    Exact math is bit-for-bit ref32; fast math stays within max-rel 1e-4
+    per conserved component of it, the state bound of the other fast-math
+    parity tests."""
+    M, N, steps = 510, 381, 40
+    s = ausm.Solver(M, N)
+    s.update_grid_boundaries(ausm.naca4_boundary(M, code=2412, size=0.3, x_offset=-0.7, exponent=1.4),
+                             ausm.rect_boundary(M, 1.5))
+    s.prepare_grid(iters=2000)
+    s.set_flow(2.0, 0.0)
+    s.set_scheme(AUSMUP, VANALBADA, 0.0, 0.75)
+    s.set_cfl(0.5, 1.0, 1)
+    vtx = s.vertices()
+    s.prepare_state(freestream=True)
+    info = s.step(steps)
+    q = s.state()
+    ref = Solver(32, make_params(2.0, 0.0, cfl=0.5, maxdt=1.0), AUSMUP, VANALBADA, vtx, omp=True)
+    ref.step(steps)
+    assert np.array_equal(q[1:N + 1], ref.state[1:N + 1])
+    assert info["dt"] == ref.last_dt
+    s.set_math("fast")
+    s.prepare_state(freestream=True)
+    s.step(steps)
+    qf = s.state().astype(np.float64)[1:N + 1]
+    r = ref.state.astype(np.float64)[1:N + 1]
+    rel = (np.abs(qf - r).max(axis=(0, 1)) / np.abs(r).max(axis=(0, 1))).max()
+    print(f"510x381 fast vs ref32 after {steps} steps: max-rel {rel:.2e}")
+    assert rel < 1e-4
     s.close()
