@@ -47,11 +47,40 @@ __device__ __forceinline__ PrimT<R> from_right(PrimT<R> w) {
 template <typename R>
 __device__ __forceinline__ R nan_max(R m, R x) { return (x > m || x != x) ? x : m; }
 
+// DPP move with a row-level control; lanes whose source is outside the
+// pattern keep their own value (old = src), the identity of a max.
+template <int CTRL>
+__device__ __forceinline__ float dpp_keep(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_keep(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
+// NaN-propagating max over the wavefront, in registers: row_shr:1,2,4,8
+// leave each 16-lane row's max in its lane 15, row_bcast:15 and
+// row_bcast:31 carry rows 0-2 up to lane 63, which is broadcast.  Max is
+// exact, so the result does not depend on the combining order.
 template <typename R>
 __device__ __forceinline__ R wave_max(R v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = nan_max(v, __shfl_xor(v, o));
-  return v;
+  v = nan_max(v, dpp_keep<0x111>(v));  // row_shr:1
+  v = nan_max(v, dpp_keep<0x112>(v));  // row_shr:2
+  v = nan_max(v, dpp_keep<0x114>(v));  // row_shr:4
+  v = nan_max(v, dpp_keep<0x118>(v));  // row_shr:8
+  v = nan_max(v, dpp_keep<0x142>(v));  // row_bcast:15
+  v = nan_max(v, dpp_keep<0x143>(v));  // row_bcast:31
+  return lane63(v);
 }
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
