@@ -24,14 +24,16 @@
 
 namespace ausm {
 
-template <int SCHEME, bool FAST, int TY>
+// LIM: the limiter as a compile-time constant (the fast path's van Albada,
+// the reference's own reconstruction), or -1 for the run-time choice.
+template <int SCHEME, bool FAST, int TY, int LIM = -1>
 __global__ __launch_bounds__(256) void stage_kernel(const StageArgs a) {
   const DevParams P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
   const DtAhead<float> ahead = stage_dt_load(a);
-  const float ws_acc = stage_tile<SCHEME, FAST, TY>(
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, false, LIM>(
       P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
@@ -70,17 +72,25 @@ int stage_blocks(int M, int N, int ty) {
   return stage_tiles_x(M) * ((N + rows_per_block - 1) / rows_per_block);
 }
 
-template <int SCHEME, bool FAST>
-static hipError_t launch_ty(const StageArgs& a, int ty, hipStream_t s) {
+template <int SCHEME, bool FAST, int LIM>
+static hipError_t launch_ty_lim(const StageArgs& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 4>), dim3(nb), dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 8>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1, LIM>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2, LIM>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 4, LIM>), dim3(nb), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 8, LIM>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int SCHEME, bool FAST>
+static hipError_t launch_ty(const StageArgs& a, int ty, hipStream_t s) {
+  if constexpr (FAST) {
+    if (a.limiter == LIM_VANALBADA) return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA>(a, ty, s);
+  }
+  return launch_ty_lim<SCHEME, FAST, -1>(a, ty, s);
 }
 
 hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s) {

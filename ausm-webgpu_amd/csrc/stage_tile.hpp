@@ -167,7 +167,8 @@ __device__ __forceinline__ void st_cons(R* row, size_t PS, uint32_t boff, R a, R
 // dt_arg is the stage's dt, or a callable that returns it; a callable is
 // evaluated once, after the workgroup barrier, so that a reduction behind it
 // can overlap the tile's loads and first face.
-template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false, typename DtF = R>
+template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false, int LIM = -1,
+          typename DtF = R>
 __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
                                         int tyb, DtF dt_arg) {
   static_assert(!FAST || sizeof(R) == 4, "the fast arithmetic is FP32 only");
@@ -188,7 +189,9 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
   const uint32_t bcr = (uint32_t)colr * (uint32_t)sizeof(R);  // and its right neighbour
   auto row_ptr = [&](const R* base, int j) { return base + (size_t)j * (size_t)Pp; };
   const int j0 = (tyb * kWavesPerBlock + wid) * TY;
-  const int lim = a.limiter;
+  // LIM >= 0: the limiter fixed at compile time (no limiter switch in the
+  // face code); -1: a.limiter at run time
+  const int lim = LIM >= 0 ? LIM : a.limiter;
   const int stage = a.stage;
 
   R ws_acc = R(0);
