@@ -68,6 +68,18 @@ __device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, in
   return a;
 }
 
+// one solve's tile with its scheme; LIM as in stage_tile (compile-time
+// limiter, or -1 for the solve's own at run time)
+template <bool FAST, int TY, bool COMP, int LIM>
+__device__ __forceinline__ float batch_tile(int scheme, const DevParams& P, const StageArgs& a, int tx,
+                                            int tyb, float dt) {
+  switch (scheme) {
+    case SCHEME_SLAU2: return stage_tile<SCHEME_SLAU2, FAST, TY, float, COMP, LIM>(P, a, tx, tyb, dt);
+    case SCHEME_SLAU: return stage_tile<SCHEME_SLAU, FAST, TY, float, COMP, LIM>(P, a, tx, tyb, dt);
+    default: return stage_tile<SCHEME_AUSMUP, FAST, TY, float, COMP, LIM>(P, a, tx, tyb, dt);
+  }
+}
+
 template <bool FAST, int TY, bool COMP>
 __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
   const int n = *b.n_items;
@@ -83,12 +95,13 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
     const int tx = item.tile % S->tiles_x;
     const int tyb = item.tile / S->tiles_x;
     const float dt = fminf(P.maxdt, P.cfl / *S->maxws_cur);
+    // fast van Albada solves (the reference's reconstruction) take the code
+    // with the limiter fixed at compile time
     float ws;
-    switch (S->scheme) {
-      case SCHEME_SLAU2: ws = stage_tile<SCHEME_SLAU2, FAST, TY, float, COMP>(P, a, tx, tyb, dt); break;
-      case SCHEME_SLAU: ws = stage_tile<SCHEME_SLAU, FAST, TY, float, COMP>(P, a, tx, tyb, dt); break;
-      default: ws = stage_tile<SCHEME_AUSMUP, FAST, TY, float, COMP>(P, a, tx, tyb, dt); break;
-    }
+    if (FAST && a.limiter == LIM_VANALBADA)
+      ws = batch_tile<FAST, TY, COMP, LIM_VANALBADA>(S->scheme, P, a, tx, tyb, dt);
+    else
+      ws = batch_tile<FAST, TY, COMP, -1>(S->scheme, P, a, tx, tyb, dt);
     if (a.stage == 3 && a.refresh) block_ws_partial(ws, S->ws_part + item.tile);
   }
 }

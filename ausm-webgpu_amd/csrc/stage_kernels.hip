@@ -53,14 +53,14 @@ __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
 }
 
 // Compensated-state FP32 variant (the COMP stages of stage_tile).
-template <int SCHEME, bool FAST, int TY>
+template <int SCHEME, bool FAST, int TY, int LIM = -1>
 __global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
   const DevParams P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
   const DtAhead<float> ahead = stage_dt_load(a);
-  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true>(
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true, LIM>(
       P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
@@ -110,16 +110,24 @@ hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStr
   return hipErrorInvalidValue;
 }
 
-template <int SCHEME, bool FAST>
-static hipError_t launch_ty_comp(const StageArgs& a, int ty, hipStream_t s) {
+template <int SCHEME, bool FAST, int LIM>
+static hipError_t launch_ty_comp_lim(const StageArgs& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 1>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 2>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 4>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 1, LIM>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 2, LIM>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 4, LIM>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int SCHEME, bool FAST>
+static hipError_t launch_ty_comp(const StageArgs& a, int ty, hipStream_t s) {
+  if constexpr (FAST) {
+    if (a.limiter == LIM_VANALBADA) return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA>(a, ty, s);
+  }
+  return launch_ty_comp_lim<SCHEME, FAST, -1>(a, ty, s);
 }
 
 hipError_t launch_stage_comp(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s) {
