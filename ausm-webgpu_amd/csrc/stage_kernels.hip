@@ -26,16 +26,18 @@ namespace ausm {
 
 // LIM: the limiter as a compile-time constant (the fast path's van Albada,
 // the reference's own reconstruction), or -1 for the run-time choice.
-template <int SCHEME, bool FAST, int TY, int LIM = -1>
+// STAGE: the RK stage as a compile-time constant (1..3), or -1 for a.stage.
+template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1>
 __global__ __launch_bounds__(256) void stage_kernel(const StageArgs a) {
   const DevParams P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const DtAhead<float> ahead = stage_dt_load(a);
-  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, false, LIM>(
-      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
-  if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
+  const int stage = STAGE >= 0 ? STAGE : a.stage;
+  const DtAhead<float> ahead = stage_dt_load(a, stage);
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, false, LIM, STAGE>(
+      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
+  if (stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
 // All-FP64 variant (exact arithmetic): the same tile walk over double
@@ -46,9 +48,9 @@ __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const DtAhead<double> ahead = stage_dt_load(a);
+  const DtAhead<double> ahead = stage_dt_load(a, a.stage);
   const double ws_acc = stage_tile<SCHEME, false, TY, double>(
-      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
+      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
@@ -59,9 +61,9 @@ __global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const DtAhead<float> ahead = stage_dt_load(a);
+  const DtAhead<float> ahead = stage_dt_load(a, a.stage);
   const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true, LIM>(
-      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead); });
+      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
@@ -72,25 +74,35 @@ int stage_blocks(int M, int N, int ty) {
   return stage_tiles_x(M) * ((N + rows_per_block - 1) / rows_per_block);
 }
 
-template <int SCHEME, bool FAST, int LIM>
+template <int SCHEME, bool FAST, int LIM, int STAGE>
 static hipError_t launch_ty_lim(const StageArgs& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1, LIM>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2, LIM>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 4, LIM>), dim3(nb), dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 8, LIM>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 4, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 8, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+// fast van Albada stages 1-3 (the stepping path of the reference's
+// reconstruction) run code specialised on limiter and stage; everything
+// else (other limiters, exact math, residual evaluation) the general kernel
 template <int SCHEME, bool FAST>
 static hipError_t launch_ty(const StageArgs& a, int ty, hipStream_t s) {
   if constexpr (FAST) {
-    if (a.limiter == LIM_VANALBADA) return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA>(a, ty, s);
+    if (a.limiter == LIM_VANALBADA) {
+      switch (a.stage) {
+        case 1: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
+        case 2: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
+        case 3: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 3>(a, ty, s);
+        default: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, -1>(a, ty, s);
+      }
+    }
   }
-  return launch_ty_lim<SCHEME, FAST, -1>(a, ty, s);
+  return launch_ty_lim<SCHEME, FAST, -1, -1>(a, ty, s);
 }
 
 hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s) {

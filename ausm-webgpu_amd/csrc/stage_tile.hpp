@@ -168,7 +168,7 @@ __device__ __forceinline__ void st_cons(R* row, size_t PS, uint32_t boff, R a, R
 // evaluated once, after the workgroup barrier, so that a reduction behind it
 // can overlap the tile's loads and first face.
 template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false, int LIM = -1,
-          typename DtF = R>
+          int STAGE = -1, typename DtF = R>
 __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
                                         int tyb, DtF dt_arg) {
   static_assert(!FAST || sizeof(R) == 4, "the fast arithmetic is FP32 only");
@@ -192,7 +192,8 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
   // LIM >= 0: the limiter fixed at compile time (no limiter switch in the
   // face code); -1: a.limiter at run time
   const int lim = LIM >= 0 ? LIM : a.limiter;
-  const int stage = a.stage;
+  // STAGE >= 0: the RK stage fixed at compile time; -1: a.stage at run time
+  const int stage = STAGE >= 0 ? STAGE : a.stage;
 
   R ws_acc = R(0);
 
@@ -497,10 +498,10 @@ struct DtAhead {
   R v[kDtAhead];
 };
 template <typename R>
-__device__ __forceinline__ DtAhead<R> stage_dt_load(const StageArgsT<R>& a) {
+__device__ __forceinline__ DtAhead<R> stage_dt_load(const StageArgsT<R>& a, int stage) {
   DtAhead<R> d;
   const int lane = threadIdx.x & 63;
-  if (a.stage == 1 && a.fresh) {
+  if (stage == 1 && a.fresh) {
 #pragma unroll
     for (int k = 0; k < kDtAhead; ++k) {
       const int i = lane + k * kWave;
@@ -511,10 +512,10 @@ __device__ __forceinline__ DtAhead<R> stage_dt_load(const StageArgsT<R>& a) {
 }
 template <typename R>
 __device__ __forceinline__ R stage_dt_finish(const DevParamsT<R>& P, const StageArgsT<R>& a,
-                                             bool writer, const DtAhead<R>& d) {
+                                             bool writer, const DtAhead<R>& d, int stage) {
   const int lane = threadIdx.x & 63;
   R maxws;
-  if (a.stage == 1 && a.fresh) {
+  if (stage == 1 && a.fresh) {
     R m = R(0);
 #pragma unroll
     for (int k = 0; k < kDtAhead; ++k) m = nan_max(m, d.v[k]);
