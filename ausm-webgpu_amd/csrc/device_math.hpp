@@ -125,13 +125,18 @@ __device__ __forceinline__ R minmod_exact(R a, R b) {
 }
 
 // MUSCL on one primitive component (ausm.js:240-249)
+// component-wise a - b (the stencil differences MUSCL works on)
 template <typename R>
-__device__ __forceinline__ void muscl_exact(int limiter, R wl2, R wl, R wr, R wr2,
-                                            R& L, R& Rr) {
+__device__ __forceinline__ PrimT<R> pdiff(PrimT<R> a, PrimT<R> b) {
+  return PrimT<R>{a.r - b.r, a.u - b.u, a.v - b.v, a.p - b.p};
+}
+
+// on the stencil differences dL = wl - wl2, dF = wr - wl, dR = wr2 - wr,
+// which a caller may share between neighbouring faces
+template <typename R>
+__device__ __forceinline__ void muscl_exact(int limiter, R wl, R wr, R dL, R dF, R dR, R& L,
+                                            R& Rr) {
   const R eps = R(1e-6);
-  const R dL = wl - wl2;
-  const R dF = wr - wl;
-  const R dR = wr2 - wr;
   if (limiter == LIM_MINMOD) {
     L = wl + R(0.5) * minmod_exact(dL, dF);
     Rr = wr - R(0.5) * minmod_exact(dF, dR);
@@ -145,19 +150,20 @@ __device__ __forceinline__ void muscl_exact(int limiter, R wl2, R wl, R wr, R wr
   }
 }
 
+// b = W_L, c = W_R and the stencil differences dL = W_L - W_L2,
+// dF = W_R - W_L, dR = W_R2 - W_R
 template <int SCHEME, typename Real>
-__device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P, int limiter, PrimT<Real> a,
-                                                   PrimT<Real> b, PrimT<Real> c, PrimT<Real> d, Real nx,
-                                                   Real ny) {
-  // a = W_L2, b = W_L, c = W_R, d = W_R2
+__device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P, int limiter, PrimT<Real> b,
+                                                   PrimT<Real> c, PrimT<Real> dL, PrimT<Real> dF,
+                                                   PrimT<Real> dR, Real nx, Real ny) {
   const Real gamma = P.gamma;
   const Real gammaMinus1 = gamma - Real(1.0);
   PrimT<Real> L = b, R = c;
   if (limiter != LIM_NONE) {
-    muscl_exact(limiter, a.r, b.r, c.r, d.r, L.r, R.r);
-    muscl_exact(limiter, a.u, b.u, c.u, d.u, L.u, R.u);
-    muscl_exact(limiter, a.v, b.v, c.v, d.v, L.v, R.v);
-    muscl_exact(limiter, a.p, b.p, c.p, d.p, L.p, R.p);
+    muscl_exact(limiter, b.r, c.r, dL.r, dF.r, dR.r, L.r, R.r);
+    muscl_exact(limiter, b.u, c.u, dL.u, dF.u, dR.u, L.u, R.u);
+    muscl_exact(limiter, b.v, c.v, dL.v, dF.v, dR.v, L.v, R.v);
+    muscl_exact(limiter, b.p, c.p, dL.p, dF.p, dR.p, L.p, R.p);
   }
   const Real u_L = L.u * nx + L.v * ny;
   const Real u_R = R.u * nx + R.v * ny;
@@ -253,12 +259,9 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
 // ---------------------------------------------------------------------------
 // Fast face flux: the same scheme with shared reciprocals and contraction.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void muscl_fast(int limiter, float wl2, float wl, float wr, float wr2,
+__device__ __forceinline__ void muscl_fast(int limiter, float wl, float wr, float dL, float dF, float dR,
                                            float& L, float& Rr) {
   const float eps = 1e-6f;
-  const float dL = wl - wl2;
-  const float dF = wr - wl;
-  const float dR = wr2 - wr;
   const float hF = 0.5f * dF;
   if (limiter == LIM_MINMOD) {
     const float mL = (dL * dF > 0.0f) ? copysignf(fminf(fabsf(dL), fabsf(dF)), dF) : 0.0f;
@@ -291,15 +294,15 @@ __device__ __forceinline__ void muscl_fast(int limiter, float wl2, float wl, flo
 }
 
 template <int SCHEME>
-__device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, Prim a, Prim b,
-                                               Prim c, Prim d, float nx, float ny) {
+__device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, Prim b, Prim c,
+                                               Prim dL, Prim dF, Prim dR, float nx, float ny) {
   const float gamma = P.gamma;
   Prim L = b, R = c;
   if (limiter != LIM_NONE) {
-    muscl_fast(limiter, a.r, b.r, c.r, d.r, L.r, R.r);
-    muscl_fast(limiter, a.u, b.u, c.u, d.u, L.u, R.u);
-    muscl_fast(limiter, a.v, b.v, c.v, d.v, L.v, R.v);
-    muscl_fast(limiter, a.p, b.p, c.p, d.p, L.p, R.p);
+    muscl_fast(limiter, b.r, c.r, dL.r, dF.r, dR.r, L.r, R.r);
+    muscl_fast(limiter, b.u, c.u, dL.u, dF.u, dR.u, L.u, R.u);
+    muscl_fast(limiter, b.v, c.v, dL.v, dF.v, dR.v, L.v, R.v);
+    muscl_fast(limiter, b.p, c.p, dL.p, dF.p, dR.p, L.p, R.p);
   }
   const float u_L = fmaf(L.u, nx, L.v * ny);
   const float u_R = fmaf(R.u, nx, R.v * ny);

@@ -88,11 +88,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// face flux from W_L = b, W_R = c and the stencil differences
+// dL = W_L - W_L2, dF = W_R - W_L, dR = W_R2 - W_R
 template <int SCHEME, bool FAST, typename R>
-__device__ __forceinline__ FluxT<R> face_flux(const DevParamsT<R>& P, int lim, PrimT<R> a,
-                                              PrimT<R> b, PrimT<R> c, PrimT<R> d, R nx, R ny) {
-  if constexpr (FAST) return face_flux_fast<SCHEME>(P, lim, a, b, c, d, nx, ny);
-  else return face_flux_exact<SCHEME>(P, lim, a, b, c, d, nx, ny);
+__device__ __forceinline__ FluxT<R> face_flux(const DevParamsT<R>& P, int lim, PrimT<R> b,
+                                              PrimT<R> c, PrimT<R> dL, PrimT<R> dF, PrimT<R> dR,
+                                              R nx, R ny) {
+  if constexpr (FAST) return face_flux_fast<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny);
+  else return face_flux_exact<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny);
 }
 
 template <bool FAST, typename R>
@@ -278,6 +281,18 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       if (j > N) return gF2;
       return to_prim(raw[k]);
     };
+    // every window slot converted once, and the differences of neighbouring
+    // slots, dd[k] = w[k] - w[k-1]: the vertical face over slots k-2..k+1
+    // takes W_L = w[k-1], W_R = w[k] and dd[k-1], dd[k], dd[k+1], so each
+    // difference serves up to three faces
+    Prim w[NR], dd[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) w[k] = prim_of(k);
+#pragma unroll
+    for (int k = 1; k < NR; ++k) dd[k] = pdiff(w[k], w[k - 1]);
+    auto vface = [&](int k, R nx, R ny) -> Flux {
+      return face_flux<SCHEME, FAST>(P, lim, w[k - 1], w[k], dd[k - 1], dd[k], dd[k + 1], nx, ny);
+    };
 
     // top face of a full tile (face index j0 + TY, window slots TY..TY+3),
     // first, so that the wave above can start from it
@@ -287,8 +302,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       R nx, ny;
       const R vt = vxr[TY], wt = vyr[TY];
       face_normal<FAST>(vt - from_right(vt), wt - from_right(wt), nx, ny, lTop);
-      const Flux G = face_flux<SCHEME, FAST>(P, lim, prim_of(TY), prim_of(TY + 1), prim_of(TY + 2),
-                                             prim_of(TY + 3), nx, ny);
+      const Flux G = vface(TY + 2, nx, ny);
       Gtop = Flux{G.f0 * lTop, G.f1 * lTop, G.f2 * lTop, G.f3 * lTop};
       s_face[wid][0][lane] = Gtop.f0;
       s_face[wid][1][lane] = Gtop.f1;
@@ -302,10 +316,6 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     else dt = dt_arg;
 
     if (active) {
-    const Prim w_m2 = prim_of(0);
-    Prim w_m1 = prim_of(1);
-    Prim w_0 = prim_of(2);
-    Prim w_p1 = prim_of(3);
     R vx0 = vxr[0], vy0 = vyr[0];
     R vx0r = from_right(vx0), vy0r = from_right(vy0);
 
@@ -320,7 +330,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     } else {
       R nx, ny;
       face_normal<FAST>(vx0 - vx0r, vy0 - vy0r, nx, ny, lD);
-      const Flux G = face_flux<SCHEME, FAST>(P, lim, w_m2, w_m1, w_0, w_p1, nx, ny);
+      const Flux G = vface(2, nx, ny);
       Gd = Flux{G.f0 * lD, G.f1 * lD, G.f2 * lD, G.f3 * lD};
     }
 
@@ -328,17 +338,19 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     for (int t = 0; t < TY; ++t) {
       const int r = j0 + t;
       if (r >= N) break;
-      const Prim w_p2 = prim_of(t + 4);
       const R vx1 = vxr[t + 1], vy1 = vyr[t + 1];
       const R vx1r = from_right(vx1), vy1r = from_right(vy1);
 
-      // horizontal face at the left of cell (i, r): stencil i-2..i+1
-      const Prim wl = from_left(w_0);
-      const Prim wl2 = from_left(wl);
-      const Prim wr = from_right(w_0);
+      // horizontal face at the left of cell (i, r): stencil i-2..i+1; each
+      // lane forms its own difference w(i) - w(i-1), its neighbours' come
+      // by shuffles
+      const Prim w0 = w[t + 2];
+      const Prim wl = from_left(w0);
+      const Prim dx = pdiff(w0, wl);
       R nxX, nyX, lL;
       face_normal<FAST>(vx1 - vx0, vy1 - vy0, nxX, nyX, lL);
-      const Flux FX = face_flux<SCHEME, FAST>(P, lim, wl2, wl, w_0, wr, nxX, nyX);
+      const Flux FX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), nxX,
+                                              nyX);
       const Flux PX{FX.f0 * lL, FX.f1 * lL, FX.f2 * lL, FX.f3 * lL};
 
       // vertical face at the top of cell (i, r): stencil r-1..r+2; the last
@@ -351,7 +363,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       } else {
         R nxY, nyY;
         face_normal<FAST>(vx1 - vx1r, vy1 - vy1r, nxY, nyY, lU);
-        const Flux FY = face_flux<SCHEME, FAST>(P, lim, w_m1, w_0, w_p1, w_p2, nxY, nyY);
+        const Flux FY = vface(t + 3, nxY, nyY);
         Gu = Flux{FY.f0 * lU, FY.f1 * lU, FY.f2 * lU, FY.f3 * lU};
       }
 
@@ -468,10 +480,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         if (lane == 0) a.res_out[r * a.tiles_x + tx] = rsum;
       }
 
-      // slide the window up one row
-      w_m1 = w_0;
-      w_0 = w_p1;
-      w_p1 = w_p2;
+      // slide the vertex window up one row
       vx0 = vx1;
       vy0 = vy1;
       vx0r = vx1r;
