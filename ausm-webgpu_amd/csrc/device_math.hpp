@@ -293,9 +293,13 @@ __device__ __forceinline__ void muscl_fast(int limiter, float wl, float wr, floa
   }
 }
 
+// Returns the flux already multiplied by the face length len; (sx, sy) =
+// len * (nx, ny) is the unnormalised edge normal, which carries the pressure
+// term without a separate scaling.
 template <int SCHEME>
 __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, Prim b, Prim c,
-                                               Prim dL, Prim dF, Prim dR, float nx, float ny) {
+                                               Prim dL, Prim dF, Prim dR, float nx, float ny,
+                                               float len, float sx, float sy) {
   const float gamma = P.gamma;
   Prim L = b, R = c;
   if (limiter != LIM_NONE) {
@@ -338,10 +342,11 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float M_o2 = fminf(1.0f, fmaxf(Mbar2, M_inf2));
     const float f_a = M_o2 * (2.0f - M_o2);
     const float rho_i = 0.5f * (L.r + R.r);
-    const float Mp1L = M_L + 1.0f, Mp1R = M_R + 1.0f;
-    const float Mm1L = M_L - 1.0f, Mm1R = M_R - 1.0f;
-    const float M2pL = 0.25f * Mp1L * Mp1L, M2pR = 0.25f * Mp1R * Mp1R;
-    const float M2mL = -0.25f * Mm1L * Mm1L, M2mR = -0.25f * Mm1R * Mm1R;
+    // M2+(M) = ((M + 1)/2)^2, M2-(M) = -((M - 1)/2)^2
+    const float hpL = fmaf(0.5f, M_L, 0.5f), hpR = fmaf(0.5f, M_R, 0.5f);
+    const float hmL = fmaf(0.5f, M_L, -0.5f), hmR = fmaf(0.5f, M_R, -0.5f);
+    const float M2pL = hpL * hpL, M2pR = hpR * hpR;
+    const float M2mL = -(hmL * hmL), M2mR = -(hmR * hmR);
     const bool supL = fabsf(M_L) >= 1.0f, supR = fabsf(M_R) >= 1.0f;
     const float M4L = supL ? fmaxf(0.0f, M_L) : M2pL * fmaf(2.0f, M2mL, 1.0f);
     const float M4R = supR ? fminf(0.0f, M_R) : M2mR * fmaf(-2.0f, M2pR, 1.0f);
@@ -388,11 +393,12 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     mdot = 0.5f * fmaf(conv, 1.0f - g, -(X * ia) * delta_p);
   }
   const bool left = mdot > 0.0f;
+  const float ml = mdot * len;
   Flux F;
-  F.f0 = mdot;
-  F.f1 = fmaf(left ? L.u : R.u, mdot, nx * P_i);
-  F.f2 = fmaf(left ? L.v : R.v, mdot, ny * P_i);
-  F.f3 = (left ? h_L : h_R) * mdot;
+  F.f0 = ml;
+  F.f1 = fmaf(left ? L.u : R.u, ml, sx * P_i);
+  F.f2 = fmaf(left ? L.v : R.v, ml, sy * P_i);
+  F.f3 = (left ? h_L : h_R) * ml;
   return F;
 }
 

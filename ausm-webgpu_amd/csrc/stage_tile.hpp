@@ -88,14 +88,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// face flux from W_L = b, W_R = c and the stencil differences
-// dL = W_L - W_L2, dF = W_R - W_L, dR = W_R2 - W_R
+// Face flux times the face length: W_L = b, W_R = c, the stencil differences
+// dL = W_L - W_L2, dF = W_R - W_L, dR = W_R2 - W_R, and the face's edge
+// vector (dx, dy) with its unit normal (nx, ny) = rot90(d)/len.  Exact math
+// scales the unit-normal flux by len afterwards, as the reference's residual
+// does; fast math folds len into the flux.
 template <int SCHEME, bool FAST, typename R>
 __device__ __forceinline__ FluxT<R> face_flux(const DevParamsT<R>& P, int lim, PrimT<R> b,
                                               PrimT<R> c, PrimT<R> dL, PrimT<R> dF, PrimT<R> dR,
-                                              R nx, R ny) {
-  if constexpr (FAST) return face_flux_fast<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny);
-  else return face_flux_exact<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny);
+                                              R dx, R dy, R nx, R ny, R len) {
+  if constexpr (FAST) {
+    return face_flux_fast<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny, len, -dy, dx);
+  } else {
+    const FluxT<R> G = face_flux_exact<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny);
+    return FluxT<R>{G.f0 * len, G.f1 * len, G.f2 * len, G.f3 * len};
+  }
 }
 
 template <bool FAST, typename R>
@@ -151,7 +158,7 @@ __device__ __forceinline__ T ld_row(const T* row, uint32_t boff) {
 }
 template <typename T>
 __device__ __forceinline__ void st_row(T* row, uint32_t boff, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(row) + boff) = v;
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(row) + boff));
 }
 // the four planes of one state row at one lane
 template <typename R>
@@ -290,8 +297,12 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     for (int k = 0; k < NR; ++k) w[k] = prim_of(k);
 #pragma unroll
     for (int k = 1; k < NR; ++k) dd[k] = pdiff(w[k], w[k - 1]);
-    auto vface = [&](int k, R nx, R ny) -> Flux {
-      return face_flux<SCHEME, FAST>(P, lim, w[k - 1], w[k], dd[k - 1], dd[k], dd[k + 1], nx, ny);
+    // vertical face over slots k-2..k+1 with edge (ex, ey), times its length
+    auto vface = [&](int k, R ex, R ey, R& len) -> Flux {
+      R nx, ny;
+      face_normal<FAST>(ex, ey, nx, ny, len);
+      return face_flux<SCHEME, FAST>(P, lim, w[k - 1], w[k], dd[k - 1], dd[k], dd[k + 1], ex, ey, nx,
+                                     ny, len);
     };
 
     // top face of a full tile (face index j0 + TY, window slots TY..TY+3),
@@ -299,11 +310,8 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     Flux Gtop{};
     R lTop = R(0);
     if (full) {
-      R nx, ny;
       const R vt = vxr[TY], wt = vyr[TY];
-      face_normal<FAST>(vt - from_right(vt), wt - from_right(wt), nx, ny, lTop);
-      const Flux G = vface(TY + 2, nx, ny);
-      Gtop = Flux{G.f0 * lTop, G.f1 * lTop, G.f2 * lTop, G.f3 * lTop};
+      Gtop = vface(TY + 2, vt - from_right(vt), wt - from_right(wt), lTop);
       s_face[wid][0][lane] = Gtop.f0;
       s_face[wid][1][lane] = Gtop.f1;
       s_face[wid][2][lane] = Gtop.f2;
@@ -328,10 +336,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
                 s_face[wid - 1][3][lane]};
       lD = s_face[wid - 1][4][lane];
     } else {
-      R nx, ny;
-      face_normal<FAST>(vx0 - vx0r, vy0 - vy0r, nx, ny, lD);
-      const Flux G = vface(2, nx, ny);
-      Gd = Flux{G.f0 * lD, G.f1 * lD, G.f2 * lD, G.f3 * lD};
+      Gd = vface(2, vx0 - vx0r, vy0 - vy0r, lD);
     }
 
 #pragma unroll
@@ -348,10 +353,10 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       const Prim wl = from_left(w0);
       const Prim dx = pdiff(w0, wl);
       R nxX, nyX, lL;
-      face_normal<FAST>(vx1 - vx0, vy1 - vy0, nxX, nyX, lL);
-      const Flux FX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), nxX,
-                                              nyX);
-      const Flux PX{FX.f0 * lL, FX.f1 * lL, FX.f2 * lL, FX.f3 * lL};
+      const R ex = vx1 - vx0, ey = vy1 - vy0;
+      face_normal<FAST>(ex, ey, nxX, nyX, lL);
+      const Flux PX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), ex,
+                                              ey, nxX, nyX, lL);
 
       // vertical face at the top of cell (i, r): stencil r-1..r+2; the last
       // row of a tile (reached only when the tile is full) has it already
@@ -361,10 +366,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         Gu = Gtop;
         lU = lTop;
       } else {
-        R nxY, nyY;
-        face_normal<FAST>(vx1 - vx1r, vy1 - vy1r, nxY, nyY, lU);
-        const Flux FY = vface(t + 3, nxY, nyY);
-        Gu = Flux{FY.f0 * lU, FY.f1 * lU, FY.f2 * lU, FY.f3 * lU};
+        Gu = vface(t + 3, vx1 - vx1r, vy1 - vy1r, lU);
       }
 
       // right face of cell (i, r) is the left face of cell (i+1, r)
