@@ -158,11 +158,20 @@ def cpu_baseline(vtx, M, N, target_s=15.0):
     t1 = run(2) / 2
     n = max(2, min(2000, int(target_s / max(t1, 1e-6))))
     dt = run(n)
+    # the same on one thread, a shorter bounded sample
+    B.base32_set_threads.argtypes = [ci]
+    B.base32_set_threads(1)
+    t1s = run(1)
+    n1 = max(1, min(200, int(target_s / 3 / max(t1s, 1e-6))))
+    dt1 = run(n1)
+    B.base32_set_threads(threads)
     return {"value": M * N * n / dt, "unit": "cell-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{n} steps of the 512x384 workload (ref32 oracle, 15 unfused passes per "
                       f"step, OpenMP over rows, {threads} threads, {dt:.1f} s)",
-            "steps_per_s": n / dt}
+            "steps_per_s": n / dt,
+            "single_thread": {"value": M * N * n1 / dt1, "unit": "cell-steps/s", "cores": 1,
+                              "sample": f"{n1} steps, {dt1:.1f} s"}}
 
 
 def probe(ausm, ev, M, N, args):
