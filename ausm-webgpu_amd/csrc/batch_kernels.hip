@@ -68,19 +68,24 @@ __device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, in
   return a;
 }
 
-// one solve's tile with its scheme; LIM as in stage_tile (compile-time
-// limiter, or -1 for the solve's own at run time)
-template <bool FAST, int TY, bool COMP, int LIM>
+// one solve's tile with its scheme; LIM and STAGE as in stage_tile
+// (compile-time limiter / RK stage, or -1 for the run-time value)
+template <bool FAST, int TY, bool COMP, int LIM, int STAGE>
 __device__ __forceinline__ float batch_tile(int scheme, const DevParams& P, const StageArgs& a, int tx,
                                             int tyb, float dt) {
   switch (scheme) {
-    case SCHEME_SLAU2: return stage_tile<SCHEME_SLAU2, FAST, TY, float, COMP, LIM>(P, a, tx, tyb, dt);
-    case SCHEME_SLAU: return stage_tile<SCHEME_SLAU, FAST, TY, float, COMP, LIM>(P, a, tx, tyb, dt);
-    default: return stage_tile<SCHEME_AUSMUP, FAST, TY, float, COMP, LIM>(P, a, tx, tyb, dt);
+    case SCHEME_SLAU2:
+      return stage_tile<SCHEME_SLAU2, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, dt);
+    case SCHEME_SLAU:
+      return stage_tile<SCHEME_SLAU, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, dt);
+    default:
+      return stage_tile<SCHEME_AUSMUP, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, dt);
   }
 }
 
-template <bool FAST, int TY, bool COMP>
+// STAGE: the launch's RK stage (1..3) at compile time, or -1 (the schedule
+// tuner's stage-2-from-Q^n launches)
+template <bool FAST, int TY, bool COMP, int STAGE>
 __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
   const int n = *b.n_items;
   const int g = xcd_remap(blockIdx.x, gridDim.x);
@@ -90,7 +95,7 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
   for (int it = lo; it < hi; ++it) {
     const BatchItem item = b.items[it];
     const SolveDev* S = b.solves + item.solve;
-    const StageArgs a = batch_args(*S, b.stage, b.refresh, b.norm);
+    const StageArgs a = batch_args(*S, STAGE > 0 ? STAGE : b.stage, b.refresh, b.norm);
     const DevParams P = S->par;
     const int tx = item.tile % S->tiles_x;
     const int tyb = item.tile / S->tiles_x;
@@ -99,9 +104,9 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
     // with the limiter fixed at compile time
     float ws;
     if (FAST && a.limiter == LIM_VANALBADA)
-      ws = batch_tile<FAST, TY, COMP, LIM_VANALBADA>(S->scheme, P, a, tx, tyb, dt);
+      ws = batch_tile<FAST, TY, COMP, LIM_VANALBADA, STAGE>(S->scheme, P, a, tx, tyb, dt);
     else
-      ws = batch_tile<FAST, TY, COMP, -1>(S->scheme, P, a, tx, tyb, dt);
+      ws = batch_tile<FAST, TY, COMP, -1, STAGE>(S->scheme, P, a, tx, tyb, dt);
     if (a.stage == 3 && a.refresh) block_ws_partial(ws, S->ws_part + item.tile);
   }
 }
@@ -160,15 +165,25 @@ __global__ void batch_seed_ws_kernel(const float* part, int n_part, float* ws_pa
   for (int i = threadIdx.x; i < n_ws; i += 64) ws_part[i] = (i == 0) ? m : 0.0f;
 }
 
-template <bool FAST, bool COMP>
-static hipError_t launch_bt(const BatchArgs& b, int ty, int blocks, hipStream_t s) {
+template <bool FAST, bool COMP, int STAGE>
+static hipError_t launch_bt_stage(const BatchArgs& b, int ty, int blocks, hipStream_t s) {
   switch (ty) {
-    case 1: hipLaunchKernelGGL((batch_stage_kernel<FAST, 1, COMP>), dim3(blocks), dim3(256), 0, s, b); break;
-    case 2: hipLaunchKernelGGL((batch_stage_kernel<FAST, 2, COMP>), dim3(blocks), dim3(256), 0, s, b); break;
-    case 4: hipLaunchKernelGGL((batch_stage_kernel<FAST, 4, COMP>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 1: hipLaunchKernelGGL((batch_stage_kernel<FAST, 1, COMP, STAGE>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 2: hipLaunchKernelGGL((batch_stage_kernel<FAST, 2, COMP, STAGE>), dim3(blocks), dim3(256), 0, s, b); break;
+    case 4: hipLaunchKernelGGL((batch_stage_kernel<FAST, 4, COMP, STAGE>), dim3(blocks), dim3(256), 0, s, b); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <bool FAST, bool COMP>
+static hipError_t launch_bt(const BatchArgs& b, int ty, int blocks, hipStream_t s) {
+  switch (b.stage) {
+    case 1: return launch_bt_stage<FAST, COMP, 1>(b, ty, blocks, s);
+    case 2: return launch_bt_stage<FAST, COMP, 2>(b, ty, blocks, s);
+    case 3: return launch_bt_stage<FAST, COMP, 3>(b, ty, blocks, s);
+    default: return launch_bt_stage<FAST, COMP, -1>(b, ty, blocks, s);
+  }
 }
 
 hipError_t launch_batch_stage(const BatchArgs& b, int fast, int comp, int ty, int blocks,
