@@ -72,14 +72,14 @@ __device__ __forceinline__ StageArgs batch_args(const SolveDev& S, int stage, in
 // (compile-time limiter / RK stage, or -1 for the run-time value)
 template <bool FAST, int TY, bool COMP, int LIM, int STAGE>
 __device__ __forceinline__ float batch_tile(int scheme, const DevParams& P, const StageArgs& a, int tx,
-                                            int tyb, float dt) {
+                                            int tyb, FaceLds<float>& s_face, float dt) {
   switch (scheme) {
     case SCHEME_SLAU2:
-      return stage_tile<SCHEME_SLAU2, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, dt);
+      return stage_tile<SCHEME_SLAU2, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, s_face, dt);
     case SCHEME_SLAU:
-      return stage_tile<SCHEME_SLAU, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, dt);
+      return stage_tile<SCHEME_SLAU, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, s_face, dt);
     default:
-      return stage_tile<SCHEME_AUSMUP, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, dt);
+      return stage_tile<SCHEME_AUSMUP, FAST, TY, float, COMP, LIM, STAGE>(P, a, tx, tyb, s_face, dt);
   }
 }
 
@@ -87,6 +87,7 @@ __device__ __forceinline__ float batch_tile(int scheme, const DevParams& P, cons
 // tuner's stage-2-from-Q^n launches)
 template <bool FAST, int TY, bool COMP, int STAGE>
 __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
+  __shared__ FaceLds<float> s_face;  // one buffer for every inlined tile variant
   const int n = *b.n_items;
   const int g = xcd_remap(blockIdx.x, gridDim.x);
   const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
@@ -104,9 +105,9 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
     // with the limiter fixed at compile time
     float ws;
     if (FAST && a.limiter == LIM_VANALBADA)
-      ws = batch_tile<FAST, TY, COMP, LIM_VANALBADA, STAGE>(S->scheme, P, a, tx, tyb, dt);
+      ws = batch_tile<FAST, TY, COMP, LIM_VANALBADA, STAGE>(S->scheme, P, a, tx, tyb, s_face, dt);
     else
-      ws = batch_tile<FAST, TY, COMP, -1, STAGE>(S->scheme, P, a, tx, tyb, dt);
+      ws = batch_tile<FAST, TY, COMP, -1, STAGE>(S->scheme, P, a, tx, tyb, s_face, dt);
     if (a.stage == 3 && a.refresh) block_ws_partial(ws, S->ws_part + item.tile);
   }
 }

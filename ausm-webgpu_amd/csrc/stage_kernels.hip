@@ -35,8 +35,9 @@ __global__ __launch_bounds__(256) void stage_kernel(const StageArgs a) {
   const int tyb = tile / a.tiles_x;
   const int stage = STAGE >= 0 ? STAGE : a.stage;
   const DtAhead<float> ahead = stage_dt_load(a, stage);
+  __shared__ FaceLds<float> s_face;
   const float ws_acc = stage_tile<SCHEME, FAST, TY, float, false, LIM, STAGE>(
-      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
+      P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
   if (stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
@@ -49,8 +50,9 @@ __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
   const DtAhead<double> ahead = stage_dt_load(a, a.stage);
+  __shared__ FaceLds<double> s_face;
   const double ws_acc = stage_tile<SCHEME, false, TY, double>(
-      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
+      P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
@@ -62,8 +64,9 @@ __global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
   const DtAhead<float> ahead = stage_dt_load(a, a.stage);
+  __shared__ FaceLds<float> s_face;
   const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true, LIM>(
-      P, a, tx, tyb, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
+      P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
   if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 

@@ -174,13 +174,19 @@ __device__ __forceinline__ void st_cons(R* row, size_t PS, uint32_t boff, R a, R
   st_row(row + 3 * PS, boff, d);
 }
 
+// The workgroup's LDS buffer for the faces its wavefronts hand up
+// (kWavesPerBlock x 5 x 64 values); a kernel declares one and passes it to
+// every tile it runs.
+template <typename R>
+using FaceLds = R[kWavesPerBlock][5][kWave];
+
 // dt_arg is the stage's dt, or a callable that returns it; a callable is
 // evaluated once, after the workgroup barrier, so that a reduction behind it
 // can overlap the tile's loads and first face.
 template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false, int LIM = -1,
           int STAGE = -1, typename DtF = R>
 __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
-                                        int tyb, DtF dt_arg) {
+                                        int tyb, FaceLds<R>& s_face, DtF dt_arg) {
   static_assert(!FAST || sizeof(R) == 4, "the fast arithmetic is FP32 only");
   using Prim = PrimT<R>;
   using Cons = ConsT<R>;
@@ -213,8 +219,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
   const bool active = j0 < N;
   const bool full = j0 + TY <= N;  // all TY rows inside the grid
   // vertical face between this workgroup's waves: wave w's top face is wave
-  // w+1's bottom face, computed once and passed through LDS
-  __shared__ R s_face[kWavesPerBlock][5][kWave];
+  // w+1's bottom face, computed once and passed through s_face
   {
     const R* __restrict__ qin = a.qin;
     // ---- prefetch: every row the tile reads, issued before any arithmetic.
