@@ -2,8 +2,8 @@
 
 Every translation unit is compiled with:
   -ffp-contract=off   the exact-math kernels follow the reference expression
-                      by expression; the fast-math code enables contraction
-                      locally with `#pragma clang fp contract(fast)`
+                      by expression; the fast-math code writes its fused
+                      multiply-adds explicitly (fmaf)
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       no packed FP32 instructions
 """
@@ -30,9 +30,15 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=
          "-Wno-unused-result", "-I" + INCLUDE, "-I" + CSRC]
 
 
+# per-file additions: the single-grid stage kernels are scheduled for
+# instruction-level parallelism (1-3 % faster across sizes, 7 % for exact
+# math); the persistent batched kernel is not (3 % slower with it)
+EXTRA_FLAGS = {"stage_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+
+
 def _compile(src):
     obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + EXTRA_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")
