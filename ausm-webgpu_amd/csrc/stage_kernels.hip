@@ -57,17 +57,18 @@ __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
 }
 
 // Compensated-state FP32 variant (the COMP stages of stage_tile).
-template <int SCHEME, bool FAST, int TY, int LIM = -1>
+template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1>
 __global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
   const DevParams P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const DtAhead<float> ahead = stage_dt_load(a, a.stage);
+  const int stage = STAGE >= 0 ? STAGE : a.stage;
+  const DtAhead<float> ahead = stage_dt_load(a, stage);
   __shared__ FaceLds<float> s_face;
-  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true, LIM>(
-      P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
-  if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true, LIM, STAGE>(
+      P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
+  if (stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
 int stage_tiles_x(int M) { return (M + kOwn - 1) / kOwn; }
@@ -90,19 +91,18 @@ static hipError_t launch_ty_lim(const StageArgs& a, int ty, hipStream_t s) {
   return hipGetLastError();
 }
 
-// fast van Albada stages 1-3 (the stepping path of the reference's
-// reconstruction) run code specialised on limiter and stage; everything
-// else (other limiters, exact math, residual evaluation) the general kernel
+// van Albada stages 1-3 (the stepping path of the reference's
+// reconstruction, fast and exact math) run code specialised on limiter and
+// stage; everything else (other limiters, residual evaluation) the general
+// kernel
 template <int SCHEME, bool FAST>
 static hipError_t launch_ty(const StageArgs& a, int ty, hipStream_t s) {
-  if constexpr (FAST) {
-    if (a.limiter == LIM_VANALBADA) {
-      switch (a.stage) {
-        case 1: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
-        case 2: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
-        case 3: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 3>(a, ty, s);
-        default: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, -1>(a, ty, s);
-      }
+  if (a.limiter == LIM_VANALBADA) {
+    switch (a.stage) {
+      case 1: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
+      case 2: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
+      case 3: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 3>(a, ty, s);
+      default: break;
     }
   }
   return launch_ty_lim<SCHEME, FAST, -1, -1>(a, ty, s);
@@ -125,24 +125,32 @@ hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStr
   return hipErrorInvalidValue;
 }
 
-template <int SCHEME, bool FAST, int LIM>
+template <int SCHEME, bool FAST, int LIM, int STAGE>
 static hipError_t launch_ty_comp_lim(const StageArgs& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 1, LIM>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 2, LIM>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 4, LIM>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 1, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 2, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 4, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+// fast van Albada stages 1-3 specialised as in launch_ty
 template <int SCHEME, bool FAST>
 static hipError_t launch_ty_comp(const StageArgs& a, int ty, hipStream_t s) {
   if constexpr (FAST) {
-    if (a.limiter == LIM_VANALBADA) return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA>(a, ty, s);
+    if (a.limiter == LIM_VANALBADA) {
+      switch (a.stage) {
+        case 1: return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
+        case 2: return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
+        case 3: return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA, 3>(a, ty, s);
+        default: break;
+      }
+    }
   }
-  return launch_ty_comp_lim<SCHEME, FAST, -1>(a, ty, s);
+  return launch_ty_comp_lim<SCHEME, FAST, -1, -1>(a, ty, s);
 }
 
 hipError_t launch_stage_comp(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s) {
