@@ -3,19 +3,25 @@
 // One launch replaces the reference's five passes of a stage (boundary,
 // vertical flux, horizontal flux, residual, stage update:
 // src/main.js:744-776, src/shaders/ausm.js:23-528):
-//   * each wavefront owns a strip of 60 cells around i (64 lanes, two halo
-//     lanes each side) and TY cells up j, and walks it row by row;
-//   * a lane keeps the primitive states of rows r-1..r+2 of its column in
-//     registers, so each cell is loaded and converted once per stage;
+//   * a workgroup is four wavefronts stacked in j; each wavefront owns a
+//     strip of 60 cells around i (64 lanes, two halo lanes each side) and TY
+//     rows up j, and loads its whole window (rows j0-2 .. j0+TY+1) before any
+//     arithmetic;
+//   * each window row is converted to primitives once, and the differences
+//     of neighbouring rows that MUSCL works on are formed once and shared by
+//     the faces that use them;
 //   * i-neighbours (MUSCL stencil, the right face's flux, the i+1 vertex)
-//     come from the adjacent lanes by wavefront shuffles;
+//     come from the adjacent lanes by DPP wavefront shifts;
+//   * a wavefront's top face is the bottom face of the wavefront above: it
+//     is computed once and passed up through LDS;
 //   * ghost rows are never stored: the body and far-field ghosts are formed
 //     on the fly from interior rows 1 and N when the window reaches them;
-//   * each face flux is evaluated once and multiplied by its length once;
-//     the residual and the RK update follow in registers;
+//   * each face flux is evaluated once, with its length; the residual and
+//     the RK update follow in registers;
 //   * stage 3 also forms the CFL wave speed of the new state (ausm.js:617-630)
 //     and a per-workgroup max (no atomics); the next stage 1 reduces those
-//     maxima itself, so dt needs no extra launch and is bitwise deterministic;
+//     maxima itself (loaded at entry, reduced after the first face), so dt
+//     needs no extra launch and is bitwise deterministic;
 //   * stage 1 writes FP64 sums of the squared density residual per (row,
 //     60-column strip) for the convergence norm, so the norm does not depend
 //     on the tile shape.

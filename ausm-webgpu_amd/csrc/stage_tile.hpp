@@ -218,8 +218,6 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
   // but compute and store nothing.
   const bool active = j0 < N;
   const bool full = j0 + TY <= N;  // all TY rows inside the grid
-  // vertical face between this workgroup's waves: wave w's top face is wave
-  // w+1's bottom face, computed once and passed through s_face
   {
     const R* __restrict__ qin = a.qin;
     // ---- prefetch: every row the tile reads, issued before any arithmetic.
@@ -311,7 +309,8 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     };
 
     // top face of a full tile (face index j0 + TY, window slots TY..TY+3),
-    // first, so that the wave above can start from it
+    // first: it is the bottom face of the wave above, which takes it from
+    // s_face after the barrier
     Flux Gtop{};
     R lTop = R(0);
     if (full) {
@@ -329,173 +328,173 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     else dt = dt_arg;
 
     if (active) {
-    R vx0 = vxr[0], vy0 = vyr[0];
-    R vx0r = from_right(vx0), vy0r = from_right(vy0);
+      R vx0 = vxr[0], vy0 = vyr[0];
+      R vx0r = from_right(vx0), vy0r = from_right(vy0);
 
-    // bottom face of row j0: vertical flux at face index j0 (ausm.js:191-205);
-    // above the first wave it is the wave below's top face (its tile is full)
-    Flux Gd;
-    R lD;
-    if (wid > 0) {
-      Gd = Flux{s_face[wid - 1][0][lane], s_face[wid - 1][1][lane], s_face[wid - 1][2][lane],
-                s_face[wid - 1][3][lane]};
-      lD = s_face[wid - 1][4][lane];
-    } else {
-      Gd = vface(2, vx0 - vx0r, vy0 - vy0r, lD);
-    }
-
-#pragma unroll
-    for (int t = 0; t < TY; ++t) {
-      const int r = j0 + t;
-      if (r >= N) break;
-      const R vx1 = vxr[t + 1], vy1 = vyr[t + 1];
-      const R vx1r = from_right(vx1), vy1r = from_right(vy1);
-
-      // horizontal face at the left of cell (i, r): stencil i-2..i+1; each
-      // lane forms its own difference w(i) - w(i-1), its neighbours' come
-      // by shuffles
-      const Prim w0 = w[t + 2];
-      const Prim wl = from_left(w0);
-      const Prim dx = pdiff(w0, wl);
-      R nxX, nyX, lL;
-      const R ex = vx1 - vx0, ey = vy1 - vy0;
-      face_normal<FAST>(ex, ey, nxX, nyX, lL);
-      const Flux PX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), ex,
-                                              ey, nxX, nyX, lL);
-
-      // vertical face at the top of cell (i, r): stencil r-1..r+2; the last
-      // row of a tile (reached only when the tile is full) has it already
-      R lU;
-      Flux Gu;
-      if (t == TY - 1) {
-        Gu = Gtop;
-        lU = lTop;
+      // bottom face of row j0: vertical flux at face index j0 (ausm.js:191-205);
+      // above the first wave it is the wave below's top face (its tile is full)
+      Flux Gd;
+      R lD;
+      if (wid > 0) {
+        Gd = Flux{s_face[wid - 1][0][lane], s_face[wid - 1][1][lane], s_face[wid - 1][2][lane],
+                  s_face[wid - 1][3][lane]};
+        lD = s_face[wid - 1][4][lane];
       } else {
-        Gu = vface(t + 3, vx1 - vx1r, vy1 - vy1r, lU);
+        Gd = vface(2, vx0 - vx0r, vy0 - vy0r, lD);
       }
 
-      // right face of cell (i, r) is the left face of cell (i+1, r)
-      const Flux PR{from_right(PX.f0), from_right(PX.f1), from_right(PX.f2), from_right(PX.f3)};
-      const R lR = from_right(lL);
+  #pragma unroll
+      for (int t = 0; t < TY; ++t) {
+        const int r = j0 + t;
+        if (r >= N) break;
+        const R vx1 = vxr[t + 1], vy1 = vyr[t + 1];
+        const R vx1r = from_right(vx1), vy1r = from_right(vy1);
 
-      // quad area (shaders/grid.js:140)
-      const R area = R(0.5) * fabs((vx1r - vx0) * (vy1 - vy0r) - (vx1 - vx0r) * (vy1r - vy0));
+        // horizontal face at the left of cell (i, r): stencil i-2..i+1; each
+        // lane forms its own difference w(i) - w(i-1), its neighbours' come
+        // by shuffles
+        const Prim w0 = w[t + 2];
+        const Prim wl = from_left(w0);
+        const Prim dx = pdiff(w0, wl);
+        R nxX, nyX, lL;
+        const R ex = vx1 - vx0, ey = vy1 - vy0;
+        face_normal<FAST>(ex, ey, nxX, nyX, lL);
+        const Flux PX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), ex,
+                                                ey, nxX, nyX, lL);
 
-      // residual dQ/dt = -(F_R l_R - F_L l_L + G_U l_U - G_D l_D) / A  (ausm.js:434)
-      R res[4];
-      {
-        const R s0 = ((PR.f0 - PX.f0) + Gu.f0) - Gd.f0;
-        const R s1 = ((PR.f1 - PX.f1) + Gu.f1) - Gd.f1;
-        const R s2 = ((PR.f2 - PX.f2) + Gu.f2) - Gd.f2;
-        const R s3 = ((PR.f3 - PX.f3) + Gu.f3) - Gd.f3;
-        if constexpr (FAST) {
-          const float ia = __builtin_amdgcn_rcpf(area);
-          res[0] = -s0 * ia;
-          res[1] = -s1 * ia;
-          res[2] = -s2 * ia;
-          res[3] = -s3 * ia;
+        // vertical face at the top of cell (i, r): stencil r-1..r+2; the last
+        // row of a tile (reached only when the tile is full) has it already
+        R lU;
+        Flux Gu;
+        if (t == TY - 1) {
+          Gu = Gtop;
+          lU = lTop;
         } else {
-          res[0] = (-s0) / area;
-          res[1] = (-s1) / area;
-          res[2] = (-s2) / area;
-          res[3] = (-s3) / area;
+          Gu = vface(t + 3, vx1 - vx1r, vy1 - vy1r, lU);
         }
-      }
 
-      if (owned) {
-        R nq[4];
-        const R cq[4] = {raw[t + 2].r, raw[t + 2].mu, raw[t + 2].mv, raw[t + 2].e};
-        R dt_c = dt;
-        if (a.local_dt && stage >= 1) {
-          // local time stepping: the cell's own CFL step from Q^n, the same
-          // in all three stages
-          const Cons& q0 = stage == 1 ? raw[t + 2] : qnr[t];
-          const R ws0 = cell_wave_speed<FAST>(P, q0.r, q0.mu, q0.mv, q0.e, ((lL + lR) + lU) + lD,
-                                              area);
-          if constexpr (FAST) dt_c = fminf(P.maxdt, P.cfl * __builtin_amdgcn_rcpf(ws0));
-          else dt_c = fmin(P.maxdt, P.cfl / ws0);
-        }
-        if (stage == 0) {
-          for (int k = 0; k < 4; ++k) nq[k] = res[k];
-        } else if constexpr (COMP) {
-          R an[4];
-          if (stage == 1) {
-            // K1 = dt L(Q^n);  Q1 = hi + K1
-            for (int k = 0; k < 4; ++k) {
-              an[k] = dt_c * res[k];
-              nq[k] = cq[k] + an[k];
-            }
+        // right face of cell (i, r) is the left face of cell (i+1, r)
+        const Flux PR{from_right(PX.f0), from_right(PX.f1), from_right(PX.f2), from_right(PX.f3)};
+        const R lR = from_right(lL);
+
+        // quad area (shaders/grid.js:140)
+        const R area = R(0.5) * fabs((vx1r - vx0) * (vy1 - vy0r) - (vx1 - vx0r) * (vy1r - vy0));
+
+        // residual dQ/dt = -(F_R l_R - F_L l_L + G_U l_U - G_D l_D) / A  (ausm.js:434)
+        R res[4];
+        {
+          const R s0 = ((PR.f0 - PX.f0) + Gu.f0) - Gd.f0;
+          const R s1 = ((PR.f1 - PX.f1) + Gu.f1) - Gd.f1;
+          const R s2 = ((PR.f2 - PX.f2) + Gu.f2) - Gd.f2;
+          const R s3 = ((PR.f3 - PX.f3) + Gu.f3) - Gd.f3;
+          if constexpr (FAST) {
+            const float ia = __builtin_amdgcn_rcpf(area);
+            res[0] = -s0 * ia;
+            res[1] = -s1 * ia;
+            res[2] = -s2 * ia;
+            res[3] = -s3 * ia;
           } else {
-            const R qh[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
-            const R ac[4] = {accr[t].r, accr[t].mu, accr[t].mv, accr[t].e};
+            res[0] = (-s0) / area;
+            res[1] = (-s1) / area;
+            res[2] = (-s2) / area;
+            res[3] = (-s3) / area;
+          }
+        }
+
+        if (owned) {
+          R nq[4];
+          const R cq[4] = {raw[t + 2].r, raw[t + 2].mu, raw[t + 2].mv, raw[t + 2].e};
+          R dt_c = dt;
+          if (a.local_dt && stage >= 1) {
+            // local time stepping: the cell's own CFL step from Q^n, the same
+            // in all three stages
+            const Cons& q0 = stage == 1 ? raw[t + 2] : qnr[t];
+            const R ws0 = cell_wave_speed<FAST>(P, q0.r, q0.mu, q0.mv, q0.e, ((lL + lR) + lU) + lD,
+                                                area);
+            if constexpr (FAST) dt_c = fminf(P.maxdt, P.cfl * __builtin_amdgcn_rcpf(ws0));
+            else dt_c = fmin(P.maxdt, P.cfl / ws0);
+          }
+          if (stage == 0) {
+            for (int k = 0; k < 4; ++k) nq[k] = res[k];
+          } else if constexpr (COMP) {
+            R an[4];
+            if (stage == 1) {
+              // K1 = dt L(Q^n);  Q1 = hi + K1
+              for (int k = 0; k < 4; ++k) {
+                an[k] = dt_c * res[k];
+                nq[k] = cq[k] + an[k];
+              }
+            } else {
+              const R qh[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
+              const R ac[4] = {accr[t].r, accr[t].mu, accr[t].mv, accr[t].e};
+              if (stage == 2) {
+                // K1 + K2;  Q2 = hi + (K1 + K2)/4
+                for (int k = 0; k < 4; ++k) {
+                  an[k] = ac[k] + dt_c * res[k];
+                  nq[k] = qh[k] + R(0.25) * an[k];
+                }
+              } else {
+                // D = (K1 + K2 + 4 K3)/6 added to hi + lo by TwoSum
+                const R lw[4] = {lor[t].r, lor[t].mu, lor[t].mv, lor[t].e};
+                for (int k = 0; k < 4; ++k) {
+                  const R K = dt_c * res[k];
+                  R D;
+                  if constexpr (FAST) D = fmaf(4.0f, K, ac[k]) * (1.0f / 6.0f);
+                  else D = (ac[k] + R(4) * K) / R(6);
+                  const R sm = qh[k] + D;
+                  const R b = sm - qh[k];
+                  const R e = (qh[k] - (sm - b)) + (D - b);
+                  const R tt = lw[k] + e;
+                  const R hn = sm + tt;
+                  const R b2 = hn - sm;
+                  an[k] = (sm - (hn - b2)) + (tt - b2);  // new lo
+                  nq[k] = hn;
+                }
+              }
+            }
+            R* dst = stage == 3 ? a.lo : a.acc;
+            st_cons(dst + (size_t)r * (size_t)Pp, PS, bc, an[0], an[1], an[2], an[3]);
+          } else if (stage == 1) {
+            // Q1 = Qn + dt L(Qn)  (ausm.js:463)
+            for (int k = 0; k < 4; ++k) nq[k] = cq[k] + dt_c * res[k];
+          } else {
+            const R qn[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
             if (stage == 2) {
-              // K1 + K2;  Q2 = hi + (K1 + K2)/4
-              for (int k = 0; k < 4; ++k) {
-                an[k] = ac[k] + dt_c * res[k];
-                nq[k] = qh[k] + R(0.25) * an[k];
-              }
+              // Q2 = 3/4 Qn + 1/4 (Q1 + dt L(Q1))  (ausm.js:494)
+              for (int k = 0; k < 4; ++k) nq[k] = R(0.75) * qn[k] + R(0.25) * (cq[k] + dt_c * res[k]);
             } else {
-              // D = (K1 + K2 + 4 K3)/6 added to hi + lo by TwoSum
-              const R lw[4] = {lor[t].r, lor[t].mu, lor[t].mv, lor[t].e};
-              for (int k = 0; k < 4; ++k) {
-                const R K = dt_c * res[k];
-                R D;
-                if constexpr (FAST) D = fmaf(4.0f, K, ac[k]) * (1.0f / 6.0f);
-                else D = (ac[k] + R(4) * K) / R(6);
-                const R sm = qh[k] + D;
-                const R b = sm - qh[k];
-                const R e = (qh[k] - (sm - b)) + (D - b);
-                const R tt = lw[k] + e;
-                const R hn = sm + tt;
-                const R b2 = hn - sm;
-                an[k] = (sm - (hn - b2)) + (tt - b2);  // new lo
-                nq[k] = hn;
+              // Qn+1 = (Qn + 2 (Q2 + dt L(Q2))) / 3  (ausm.js:525)
+              if constexpr (FAST) {
+                for (int k = 0; k < 4; ++k)
+                  nq[k] = fmaf(2.0f, fmaf(dt_c, res[k], cq[k]), qn[k]) * (1.0f / 3.0f);
+              } else {
+                for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * (cq[k] + dt_c * res[k])) / R(3);
               }
             }
           }
-          R* dst = stage == 3 ? a.lo : a.acc;
-          st_cons(dst + (size_t)r * (size_t)Pp, PS, bc, an[0], an[1], an[2], an[3]);
-        } else if (stage == 1) {
-          // Q1 = Qn + dt L(Qn)  (ausm.js:463)
-          for (int k = 0; k < 4; ++k) nq[k] = cq[k] + dt_c * res[k];
-        } else {
-          const R qn[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
-          if (stage == 2) {
-            // Q2 = 3/4 Qn + 1/4 (Q1 + dt L(Q1))  (ausm.js:494)
-            for (int k = 0; k < 4; ++k) nq[k] = R(0.75) * qn[k] + R(0.25) * (cq[k] + dt_c * res[k]);
-          } else {
-            // Qn+1 = (Qn + 2 (Q2 + dt L(Q2))) / 3  (ausm.js:525)
-            if constexpr (FAST) {
-              for (int k = 0; k < 4; ++k)
-                nq[k] = fmaf(2.0f, fmaf(dt_c, res[k], cq[k]), qn[k]) * (1.0f / 3.0f);
-            } else {
-              for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * (cq[k] + dt_c * res[k])) / R(3);
-            }
+          st_cons(a.qout + (size_t)r * (size_t)Pp, PS, bc, nq[0], nq[1], nq[2], nq[3]);
+          if (stage == 3 && a.refresh) {
+            const R ws = cell_wave_speed<FAST>(P, nq[0], nq[1], nq[2], nq[3],
+                                               ((lL + lR) + lU) + lD, area);
+            ws_acc = nan_max(ws_acc, ws);
           }
         }
-        st_cons(a.qout + (size_t)r * (size_t)Pp, PS, bc, nq[0], nq[1], nq[2], nq[3]);
-        if (stage == 3 && a.refresh) {
-          const R ws = cell_wave_speed<FAST>(P, nq[0], nq[1], nq[2], nq[3],
-                                             ((lL + lR) + lU) + lD, area);
-          ws_acc = nan_max(ws_acc, ws);
+
+        if (stage == 1 && a.res_out) {
+          const double rr = owned ? (double)res[0] * (double)res[0] : 0.0;
+          const double rsum = wave_sum(rr);
+          if (lane == 0) a.res_out[r * a.tiles_x + tx] = rsum;
         }
-      }
 
-      if (stage == 1 && a.res_out) {
-        const double rr = owned ? (double)res[0] * (double)res[0] : 0.0;
-        const double rsum = wave_sum(rr);
-        if (lane == 0) a.res_out[r * a.tiles_x + tx] = rsum;
+        // slide the vertex window up one row
+        vx0 = vx1;
+        vy0 = vy1;
+        vx0r = vx1r;
+        vy0r = vy1r;
+        Gd = Gu;
+        lD = lU;
       }
-
-      // slide the vertex window up one row
-      vx0 = vx1;
-      vy0 = vy1;
-      vx0r = vx1r;
-      vy0r = vy1r;
-      Gd = Gu;
-      lD = lU;
     }
-    }  // active
   }
   // the LDS faces may be rewritten by the next tile of a persistent loop
   __syncthreads();
