@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 import ausm
-from oracle_lib import AUSMUP, SLAU2, VANALBADA, Solver
+from oracle_lib import AUSMUP, SLAU2, VANALBADA, Solver, make_params64
+from test_gpu_fp64 import perturbed_state64
 from test_gpu_parity import flow, max_rel, perturbed_state
 
 pytestmark = pytest.mark.gpu
@@ -22,14 +23,14 @@ pytestmark = pytest.mark.gpu
 TILES = [1, 2, 4, 8]
 
 
-def annulus(M, N, r0=0.5, r1=3.0):
-    """(N+1, M, 2) FP32 vertices: body row j = 0 counter-clockwise, far
-    field at j = N."""
+def annulus(M, N, r0=0.5, r1=3.0, dtype=np.float32):
+    """(N+1, M, 2) vertices: body row j = 0 counter-clockwise, far field at
+    j = N."""
     th = 2.0 * np.pi * np.arange(M) / M
     r = r0 * (r1 / r0) ** (np.arange(N + 1) / N)
     x = r[:, None] * np.cos(th)[None, :]
     y = r[:, None] * np.sin(th)[None, :]
-    return np.stack([x, y], axis=-1).astype(np.float32)
+    return np.stack([x, y], axis=-1).astype(dtype)
 
 
 def run(vtx, scheme, math_mode, tile, st, steps, fs, par):
@@ -98,3 +99,50 @@ def test_large_grid_exact_bitwise_and_fast_tiles():
     q4, _ = run(vtx, AUSMUP, "fast", 4, st, 4, fs, par)
     assert np.array_equal(q2, q4)
     assert max_rel(q2[1:N + 1], ref.state[1:N + 1]) < 1e-4
+
+
+@pytest.mark.parametrize("M,N", [(4, 4), (61, 5), (125, 17)])
+def test_shapes_fp64_and_compensated(M, N):
+    """FP64 bit-for-bit ref64 and FP32C (exact math) bit-for-bit the oracle's
+    compensated mode, hi and lo, for each tile shape of those kernels."""
+    fs = ausm.freestream(0.6, 5.0)
+    vtx64 = annulus(M, N, dtype=np.float64)
+    par64 = make_params64(fs, cfl=0.5, maxdt=1.0)
+    st64 = perturbed_state64(par64, M, N, seed=M + N)
+    ref64 = Solver(64, par64, SLAU2, VANALBADA, vtx64)
+    ref64.set_state(st64)
+    ref64.step(6)
+    vtx = annulus(M, N)
+    _, par = flow(0.6, 5.0)
+    st = perturbed_state(par, M, N, seed=M + N)
+    refc = Solver(32, par, SLAU2, VANALBADA, vtx, comp=True)
+    refc.set_state(st)
+    refc.step(6)
+    for tile in (1, 2, 4):
+        s = ausm.Solver(M, N)
+        s.set_vertices(vtx64)
+        s.set_freestream(*fs)
+        s.set_scheme(SLAU2, VANALBADA, 0.0, 0.75)
+        s.set_cfl(0.5, 1.0, 1)
+        s.set_tile(tile)
+        s.set_precision("fp64")
+        s.set_state64(st64)
+        s.step(6)
+        q = s.state64()
+        assert np.array_equal(q[1:N + 1], ref64.state[1:N + 1]), (M, N, tile)
+        s.close()
+        s = ausm.Solver(M, N)
+        s.set_vertices(vtx)
+        s.set_freestream(*fs)
+        s.set_scheme(SLAU2, VANALBADA, float(par[11]), float(par[12]))
+        s.set_cfl(float(par[10]), float(par[9]), 1)
+        s.set_math("exact")
+        s.set_tile(tile)
+        s.set_precision("fp32c")
+        s.set_state(st)
+        s.step(6)
+        assert np.array_equal(s.state()[1:N + 1], refc.state[1:N + 1]), (M, N, tile)
+        both = s.state64()[1:N + 1]
+        expect = refc.state[1:N + 1].astype(np.float64) + refc.lo[1:N + 1].astype(np.float64)
+        assert np.array_equal(both, expect), (M, N, tile)
+        s.close()
