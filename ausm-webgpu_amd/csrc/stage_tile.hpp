@@ -204,7 +204,13 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
   const uint32_t bc = (uint32_t)col * (uint32_t)sizeof(R);    // this lane's column
   const uint32_t bcr = (uint32_t)colr * (uint32_t)sizeof(R);  // and its right neighbour
   auto row_ptr = [&](const R* base, int j) { return base + (size_t)j * (size_t)Pp; };
-  const int j0 = (tyb * kWavesPerBlock + wid) * TY;
+  // row block of this wavefront among the workgroup's kWavesPerBlock: the
+  // bottom block has one face more than the others, so the block order is
+  // rotated with the tile row and the extra work does not always fall on the
+  // same SIMD of a compute unit
+  static_assert((kWavesPerBlock & (kWavesPerBlock - 1)) == 0, "rotation needs a power of two");
+  const int rb = (wid + tyb) & (kWavesPerBlock - 1);
+  const int j0 = (tyb * kWavesPerBlock + rb) * TY;
   // LIM >= 0: the limiter fixed at compile time (no limiter switch in the
   // face code); -1: a.limiter at run time
   const int lim = LIM >= 0 ? LIM : a.limiter;
@@ -316,11 +322,11 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     if (full) {
       const R vt = vxr[TY], wt = vyr[TY];
       Gtop = vface(TY + 2, vt - from_right(vt), wt - from_right(wt), lTop);
-      s_face[wid][0][lane] = Gtop.f0;
-      s_face[wid][1][lane] = Gtop.f1;
-      s_face[wid][2][lane] = Gtop.f2;
-      s_face[wid][3][lane] = Gtop.f3;
-      s_face[wid][4][lane] = lTop;
+      s_face[rb][0][lane] = Gtop.f0;
+      s_face[rb][1][lane] = Gtop.f1;
+      s_face[rb][2][lane] = Gtop.f2;
+      s_face[rb][3][lane] = Gtop.f3;
+      s_face[rb][4][lane] = lTop;
     }
     __syncthreads();
     R dt;
@@ -332,13 +338,14 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       R vx0r = from_right(vx0), vy0r = from_right(vy0);
 
       // bottom face of row j0: vertical flux at face index j0 (ausm.js:191-205);
-      // above the first wave it is the wave below's top face (its tile is full)
+      // above the workgroup's bottom row block it is the top face of the block
+      // below (whose tile is full)
       Flux Gd;
       R lD;
-      if (wid > 0) {
-        Gd = Flux{s_face[wid - 1][0][lane], s_face[wid - 1][1][lane], s_face[wid - 1][2][lane],
-                  s_face[wid - 1][3][lane]};
-        lD = s_face[wid - 1][4][lane];
+      if (rb > 0) {
+        Gd = Flux{s_face[rb - 1][0][lane], s_face[rb - 1][1][lane], s_face[rb - 1][2][lane],
+                  s_face[rb - 1][3][lane]};
+        lD = s_face[rb - 1][4][lane];
       } else {
         Gd = vface(2, vx0 - vx0r, vy0 - vy0r, lD);
       }
