@@ -95,3 +95,35 @@ def test_literal_510x381_supersonic():
     print(f"510x381 fast vs ref32 after {steps} steps: max-rel {rel:.2e}")
     assert rel < 1e-4
     s.close()
+
+
+def test_headline_fast_math_tracks_exact_over_2000_steps():
+    """The single-grid headline configuration (512 x 384 NACA 2412, Mach 2,
+    AUSM+-up + van Albada, CFL 0.5) run 2000 steps from the free stream with
+    exact and with fast math: the bow-shock solution agrees in Cl and Cd to
+    1e-4 and in the state to max-rel 1e-3 per conserved component (the
+    bound of the FP32 vs FP64 comparison above, which is how far any FP32
+    rounding choice moves this solution)."""
+    M, N, steps = 512, 384, 2000
+    s = ausm.Solver(M, N)
+    s.update_grid_boundaries(ausm.naca4_boundary(M, code=2412, size=0.3, x_offset=-0.7, exponent=1.4),
+                             ausm.rect_boundary(M, 1.5))
+    s.prepare_grid(iters=2000)
+    s.set_flow(2.0, 0.0)
+    s.set_scheme(AUSMUP, VANALBADA, 0.0, 0.75)
+    s.set_cfl(0.5, 1.0, 1)
+    out = {}
+    for mode in ("exact", "fast"):
+        s.set_math(mode)
+        s.prepare_state(freestream=True)
+        info = s.step(steps)
+        assert not info["nonfinite"]
+        out[mode] = (s.state().astype(np.float64)[1:N + 1], s.forces(ref_length=1.0))
+    (qe, fe), (qf, ff) = out["exact"], out["fast"]
+    rel = (np.abs(qf - qe).max(axis=(0, 1)) / np.abs(qe).max(axis=(0, 1))).max()
+    print(f"headline 2000 steps: Cl {fe['cl']:.6f} / {ff['cl']:.6f}, Cd {fe['cd']:.6f} / {ff['cd']:.6f}, "
+          f"state max-rel {rel:.2e}")
+    assert abs(ff["cl"] - fe["cl"]) < 1e-4
+    assert abs(ff["cd"] - fe["cd"]) < 1e-4
+    assert rel < 1e-3
+    s.close()
