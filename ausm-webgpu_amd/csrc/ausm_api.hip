@@ -228,7 +228,17 @@ StageArgs stage_args(ausm_ctx* c, int ty, int stage) {
 // passes, main.js:744-777).  `norm`: stage 1 also writes the partial sums of
 // the density-residual norm (only the last step of a call needs them).
 hipError_t enqueue_step(ausm_ctx* c, int ty, bool fresh, bool refresh, bool norm) {
-  hipError_t e = stage_launch(c, ty, 1, 0, 0, 1, fresh, false, norm);
+  hipError_t e = hipSuccess;
+  if (fresh && c->n_ws_valid > kWsInKernelMax) {
+    // large grids: one launch reduces the maxima of the last refresh into
+    // the stage's max wave speed, rather than every wavefront of stage 1
+    // reading all of them (the max is exact, so dt is the same either way)
+    e = fp64(c) ? launch_reduce_max(c->ws_part64, c->n_ws_valid, c->scal_d + 6, c->stream)
+                : launch_reduce_max(c->ws_part, c->n_ws_valid, c->scal_f, c->stream);
+    if (e != hipSuccess) return e;
+    fresh = false;
+  }
+  e = stage_launch(c, ty, 1, 0, 0, 1, fresh, false, norm);
   if (e != hipSuccess) return e;
   e = stage_launch(c, ty, 2, 1, 0, 2, false, false, false);
   if (e != hipSuccess) return e;

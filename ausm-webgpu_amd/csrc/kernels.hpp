@@ -9,6 +9,13 @@ namespace ausm {
 constexpr int kWave = 64;
 constexpr int kOwn = 60;           // columns a wavefront updates (2 halo lanes each side)
 constexpr int kWavesPerBlock = 4;  // wavefronts per stage-kernel workgroup
+// Partial wave-speed maxima a fresh stage 1 reduces itself: kDtAhead per lane
+// are loaded at kernel entry, the rest (up to kWsInKernelMax) in a loop;
+// beyond that the host reduces them once in a separate launch instead of in
+// every wavefront (at 512 x 384 with one row per wavefront, 864 maxima, the
+// in-kernel loop is the cheaper; at 1024 x 768 with two, 1632, the launch).
+constexpr int kDtAhead = 8;
+constexpr int kWsInKernelMax = 1024;
 
 // One launch of the fused RK-stage kernel.  State buffers are SoA planes
 // (rho, rho*u, rho*v, rho*E) of N interior rows with row pitch P and plane

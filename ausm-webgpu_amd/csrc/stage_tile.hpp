@@ -513,8 +513,8 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
 // first kDtAhead per lane are loaded by stage_dt_load at kernel entry, in
 // one batch that overlaps the tile's own loads, and reduced by
 // stage_dt_finish where dt is first needed (the same order of maxima as a
-// plain lane-strided loop, so dt does not depend on the split).
-constexpr int kDtAhead = 8;
+// plain lane-strided loop, so dt does not depend on the split).  The host
+// keeps this path to at most kWsInKernelMax maxima.
 template <typename R>
 struct DtAhead {
   R v[kDtAhead];
