@@ -10,6 +10,7 @@
 //     planes the stage kernel streams, with ghost rows formed as the
 //     boundary pass (ausm.js:34-120) forms them.
 #include "aux_kernels.hpp"
+#include "reduce.hpp"
 
 namespace ausm {
 
@@ -169,8 +170,7 @@ __global__ void wave_speed_kernel(const R* q, const R* vx, const R* vy, const De
 
 template <typename R>
 __global__ void reduce_max_kernel(const R* part, int n, R* out) {
-  R m = R(0);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) m = nan_max2(m, part[i]);
+  const R m = strided_max(part, n, (int)threadIdx.x, (int)blockDim.x);
   __shared__ R sm[256];
   sm[threadIdx.x] = m;
   __syncthreads();
@@ -185,8 +185,7 @@ __global__ void reduce_max_kernel(const R* part, int n, R* out) {
 // per-thread sums, then a fixed tree): deterministic for a given n
 __global__ void reduce_sum_f64_kernel(const double* part, int n, double* out) {
   __shared__ double sm[256];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  const double s = strided_sum(part, n, (int)threadIdx.x, 256);
   sm[threadIdx.x] = s;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {

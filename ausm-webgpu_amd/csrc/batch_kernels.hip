@@ -16,6 +16,7 @@
 //  * Item count and solve flags live in device memory, so one captured
 //    graph serves every batch composition.
 #include "batch_kernels.hpp"
+#include "reduce.hpp"
 #include "stage_tile.hpp"
 
 namespace ausm {
@@ -122,9 +123,7 @@ __global__ void batch_dt_kernel(const SolveDev* solves, const int* n_solves) {
   if (s >= *n_solves) return;
   const SolveDev& S = solves[s];
   if (!S.active) return;
-  float m = 0.0f;
-  for (int i = threadIdx.x; i < S.n_ws; i += 64) m = nan_max(m, S.ws_part[i]);
-  m = wave_max(m);
+  const float m = wave_max(strided_max(S.ws_part, S.n_ws, (int)threadIdx.x, 64));
   if (threadIdx.x == 0) *S.maxws_cur = m;
 }
 
@@ -140,10 +139,8 @@ __global__ void batch_norm_kernel(const SolveDev* solves, const int* n_solves, d
   __shared__ double sm[256];
   __shared__ float smw[256];
   const int n = S.N * S.tiles_x;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) acc += S.res_part[i];
-  float w = 0.0f;
-  for (int i = threadIdx.x; i < S.n_ws; i += 256) w = nan_max(w, S.ws_part[i]);
+  const double acc = strided_sum(S.res_part, n, (int)threadIdx.x, 256);
+  const float w = strided_max(S.ws_part, S.n_ws, (int)threadIdx.x, 256);
   sm[threadIdx.x] = acc;
   smw[threadIdx.x] = w;
   __syncthreads();
@@ -163,9 +160,7 @@ __global__ void batch_norm_kernel(const SolveDev* solves, const int* n_solves, d
 // a solve's initial wave-speed partials in the stage-3 layout: the max of
 // the per-block values in entry 0, zeros after it
 __global__ void batch_seed_ws_kernel(const float* part, int n_part, float* ws_part, int n_ws) {
-  float m = 0.0f;
-  for (int i = threadIdx.x; i < n_part; i += 64) m = nan_max(m, part[i]);
-  m = wave_max(m);
+  const float m = wave_max(strided_max(part, n_part, (int)threadIdx.x, 64));
   for (int i = threadIdx.x; i < n_ws; i += 64) ws_part[i] = (i == 0) ? m : 0.0f;
 }
 

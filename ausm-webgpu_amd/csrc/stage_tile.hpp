@@ -4,6 +4,7 @@
 #include <type_traits>
 
 #include "kernels.hpp"
+#include "reduce.hpp"
 
 namespace ausm {
 
@@ -541,7 +542,7 @@ __device__ __forceinline__ R stage_dt_finish(const DevParamsT<R>& P, const Stage
     R m = R(0);
 #pragma unroll
     for (int k = 0; k < kDtAhead; ++k) m = nan_max(m, d.v[k]);
-    for (int i = lane + kDtAhead * kWave; i < a.n_ws_in; i += kWave) m = nan_max(m, a.ws_in[i]);
+    m = nan_max(m, strided_max(a.ws_in, a.n_ws_in, lane + kDtAhead * kWave, kWave));
     maxws = wave_max(m);
     if (writer && threadIdx.x == 0) *a.maxws_cur = maxws;
   } else {
