@@ -112,13 +112,16 @@ def time_steps(s, ev, steps, warmup):
     return ms, wall, info
 
 
-def cpu_baseline(vtx, M, N, target_s=15.0):
+def cpu_baseline(vtx, M, N, target_s=15.0, q_gpu=None, r_gpu=None):
     """The FP32 oracle, unfused 15 passes per step, OpenMP over rows, on a
-    bounded number of steps of the same 512 x 384 workload."""
+    bounded number of steps of the same 512 x 384 workload.  With the GPU's
+    state and residual at the end of its timed run (q_gpu, r_gpu), the
+    oracle also evaluates the residual of that state: the per-step residual
+    agreement of the measured (fast-math) step with the restated WGSL."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import make_params
+    from oracle_lib import AUSMUP, VANALBADA, Oracle, make_params
 
     libp = os.path.join(ROOT, "oracle", "libbaseline.so")
     if not os.path.exists(libp):
@@ -165,8 +168,21 @@ def cpu_baseline(vtx, M, N, target_s=15.0):
     n1 = max(1, min(200, int(target_s / 3 / max(t1s, 1e-6))))
     dt1 = run(n1)
     B.base32_set_threads(threads)
+    agreement = None
+    if q_gpu is not None and r_gpu is not None:
+        r_ref = Oracle(32).residual(par, AUSMUP, VANALBADA, v, q_gpu).astype(np.float64)
+        rg = r_gpu.astype(np.float64)
+        d = np.abs(rg - r_ref).reshape(-1, 4).max(axis=0)
+        sc = np.abs(r_ref).reshape(-1, 4).max(axis=0)
+        n_ref = math.sqrt(float((r_ref[..., 0] ** 2).sum()))
+        n_gpu = math.sqrt(float((rg[..., 0] ** 2).sum()))
+        agreement = {"state": "GPU state after the timed steps",
+                     "max_rel_per_component": [float(x) for x in d / np.maximum(sc, 1e-30)],
+                     "rho_norm_rel": abs(n_gpu - n_ref) / n_ref,
+                     "bound": "field max-rel 1e-4, norm 1e-5 (tests/test_gpu_parity.py)"}
     return {"value": M * N * n / dt, "unit": "cell-steps/s", "cores": threads,
             "kind": "port",
+            "residual_agreement_vs_ref32": agreement,
             "sample": f"{n} steps of the 512x384 workload (ref32 oracle, 15 unfused passes per "
                       f"step, OpenMP over rows, {threads} threads, {dt:.1f} s)",
             "steps_per_s": n / dt,
@@ -363,6 +379,7 @@ def main():
     step_ms = ms / args.steps
     log(f"[bench] {rate:.3e} cell-steps/s, {step_ms * 1e3:.2f} us/step, wall {wall:.3f} s, "
         f"res_l2 {info['res_l2']:.3e}")
+    q_end, r_end = s.state(), s.residual()
 
     # the stage kernel's average launch duration over the timed region: the
     # HIP events above bracket exactly 3 x steps stage launches on the
@@ -391,10 +408,22 @@ def main():
         log(f"[bench] {ml}x{nl}: {literal['value']:.3e} cell-steps/s")
         sl.close()
 
+    # the same workload with SLAU2 (the reference's default scheme)
+    s.set_scheme(ausm.SLAU2, ausm.VANALBADA, 0.0, 0.75)
+    n_sl = max(1, args.steps // 5)
+    ms_sl, _, _ = time_steps(s, ev, n_sl, min(args.warmup, 100))
+    slau2_rate = M * N * n_sl / (ms_sl / 1e3)
+    log(f"[bench] SLAU2: {slau2_rate:.3e} cell-steps/s")
+    s.set_scheme(ausm.AUSMUP, ausm.VANALBADA, 0.0, 0.75)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline")
-        cpu = cpu_baseline(s.vertices(), M, N)
+        cpu = cpu_baseline(s.vertices(), M, N, q_gpu=q_end, r_gpu=r_end)
+        if cpu["residual_agreement_vs_ref32"]:
+            a = cpu["residual_agreement_vs_ref32"]
+            log(f"[bench] residual vs ref32: max-rel {max(a['max_rel_per_component']):.2e}, "
+                f"rho norm {a['rho_norm_rel']:.2e}")
         log(f"[bench] CPU: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
 
     from ausm import serve
@@ -437,6 +466,7 @@ def main():
         "pct_hbm_roofline_whole_step": 100.0 * whole_gbps / HBM_PEAK_GBPS,
         "steps_per_s": args.steps / (ms / 1e3),
         "exact_math_cell_steps_per_s": exact_rate,
+        "slau2_cell_steps_per_s": slau2_rate,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "traffic_source": "profiles/stage_kernel_traffic.json (rocprofv3 --pmc "
