@@ -357,10 +357,12 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
       M_i -= kp;
     }
     const float alpha4 = 0.75f * fmaf(5.0f * f_a, f_a, -4.0f);
-    const float P5L = supL ? (M_L >= 0.0f ? 1.0f : 0.0f)
-                           : M2pL * fmaf(alpha4 * M_L, M2mL, 2.0f - M_L);
-    const float P5R = supR ? (-M_R >= 0.0f ? 1.0f : 0.0f)
-                           : M2mR * fmaf(-alpha4 * M_R, M2pR, -2.0f - M_R);
+    // both sides evaluated, then selected
+    const float p5sL = M2pL * fmaf(alpha4 * M_L, M2mL, 2.0f - M_L);
+    const float p5sR = M2mR * fmaf(-alpha4 * M_R, M2pR, -2.0f - M_R);
+    const float stL = M_L >= 0.0f ? 1.0f : 0.0f, stR = -M_R >= 0.0f ? 1.0f : 0.0f;
+    const float P5L = supL ? stL : p5sL;
+    const float P5R = supR ? stR : p5sR;
     const float ku = P.Ku * P5L * P5R * (L.r + R.r) * f_a * a_i * (u_R - u_L);
     P_i = fmaf(P5L, L.p, P5R * R.p) - ku;
     mdot = a_i * M_i * (M_i > 0.0f ? L.r : R.r);
