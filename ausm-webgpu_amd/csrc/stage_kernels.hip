@@ -49,17 +49,18 @@ __global__ __launch_bounds__(256) void stage_kernel(const StageArgs a) {
 
 // All-FP64 variant (exact arithmetic): the same tile walk over double
 // state and vertex planes, for the FP64 comparison solve.
-template <int SCHEME, int TY>
+template <int SCHEME, int TY, int LIM = -1, int STAGE = -1>
 __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
   const DevParamsT<double> P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
   const int tyb = tile / a.tiles_x;
-  const DtAhead<double> ahead = stage_dt_load(a, a.stage);
+  const int stage = STAGE >= 0 ? STAGE : a.stage;
+  const DtAhead<double> ahead = stage_dt_load(a, stage);
   __shared__ FaceLds<double> s_face;
-  const double ws_acc = stage_tile<SCHEME, false, TY, double>(
-      P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, a.stage); });
-  if (a.stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
+  const double ws_acc = stage_tile<SCHEME, false, TY, double, false, LIM, STAGE>(
+      P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
+  if (stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
 // Compensated-state FP32 variant (the COMP stages of stage_tile).
@@ -171,16 +172,30 @@ hipError_t launch_stage_comp(const StageArgs& a, int scheme, int fast, int ty, h
   return hipErrorInvalidValue;
 }
 
-template <int SCHEME>
-static hipError_t launch_ty64(const StageArgs64& a, int ty, hipStream_t s) {
+template <int SCHEME, int LIM, int STAGE>
+static hipError_t launch_ty64_lim(const StageArgs64& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel64<SCHEME, 1>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel64<SCHEME, 2>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel64<SCHEME, 4>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel64<SCHEME, 1, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel64<SCHEME, 2, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel64<SCHEME, 4, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// van Albada stages 1-3 specialised as in launch_ty
+template <int SCHEME>
+static hipError_t launch_ty64(const StageArgs64& a, int ty, hipStream_t s) {
+  if (a.limiter == LIM_VANALBADA) {
+    switch (a.stage) {
+      case 1: return launch_ty64_lim<SCHEME, LIM_VANALBADA, 1>(a, ty, s);
+      case 2: return launch_ty64_lim<SCHEME, LIM_VANALBADA, 2>(a, ty, s);
+      case 3: return launch_ty64_lim<SCHEME, LIM_VANALBADA, 3>(a, ty, s);
+      default: break;
+    }
+  }
+  return launch_ty64_lim<SCHEME, -1, -1>(a, ty, s);
 }
 
 hipError_t launch_stage64(const StageArgs64& a, int scheme, int ty, hipStream_t s) {
