@@ -361,30 +361,38 @@ static int admit(ausm_batch* b) {
   return AUSM_OK;
 }
 
-// time one tuning stage launch over the current items
+// time one step's worth of stage launches over the current items (after
+// one untimed launch): stages 1 and 2 as a step runs them, then the stage-2
+// arithmetic from Q^n in place of stage 3.  They write only what every step
+// rewrites before it reads it (Q1, Q2 and the compensated increment), so
+// tuning changes no solve's results.
 static float time_schedule(ausm_batch* b, int ty, int bpc) {
-  BatchArgs a{b->d_solves, b->d_items, b->d_counts, 4, 0, 0};
+  const BatchArgs a1{b->d_solves, b->d_items, b->d_counts, 1, 0, 0};
+  const BatchArgs a2{b->d_solves, b->d_items, b->d_counts, 2, 0, 0};
+  const BatchArgs a4{b->d_solves, b->d_items, b->d_counts, 4, 0, 0};
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   const int blocks = b->cus * bpc;
-  for (int w = 0; w < 2; ++w) (void)launch_batch_stage(a, b->math, b->comp, ty, blocks, b->stream);
+  (void)launch_batch_stage(a4, b->math, b->comp, ty, blocks, b->stream);
+  (void)hipEventRecord(e0, b->stream);
+  for (const BatchArgs* a : {&a1, &a2, &a4})
+    (void)launch_batch_stage(*a, b->math, b->comp, ty, blocks, b->stream);
+  (void)hipEventRecord(e1, b->stream);
+  (void)hipEventSynchronize(e1);
   float ms = 1e30f;
-  for (int round = 0; round < 2; ++round) {
-    (void)hipEventRecord(e0, b->stream);
-    for (int k = 0; k < 3; ++k) (void)launch_batch_stage(a, b->math, b->comp, ty, blocks, b->stream);
-    (void)hipEventRecord(e1, b->stream);
-    (void)hipEventSynchronize(e1);
-    float t = 1e30f;
-    (void)hipEventElapsedTime(&t, e0, e1);
-    ms = std::min(ms, t);
-  }
+  (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return ms;
 }
 
-// choose (rows per wavefront, workgroups per CU) for the current mix
+static const int kBatchBpc[] = {1, 2, 4, 8};
+
+// choose (rows per wavefront, workgroups per CU) for the current mix: three
+// rounds over every candidate, a candidate's time its best round (the
+// rounds interleave the candidates, so clock ramps and queue noise do not
+// favour the ones timed last)
 static int choose_schedule(ausm_batch* b) {
   long long cells = 0;
   for (auto* s : b->table)
@@ -398,22 +406,29 @@ static int choose_schedule(ausm_batch* b) {
     if (!bpc) bpc = it->second.second;
   }
   if (!ty || !bpc) {
-    float best = 1e30f;
-    int bty = 2, bbpc = 2;
+    std::vector<std::pair<int, int>> cand;
     for (int t : kBatchTiles) {
       if (b->fixed_ty && t != b->fixed_ty) continue;
-      int r = build_items(b, t);
-      if (r != AUSM_OK) return r;
-      for (int k : {1, 2, 4}) {
-        if (b->fixed_bpc && k != b->fixed_bpc) continue;
-        const float ms = time_schedule(b, t, k);
-        if (ms < best) {
-          best = ms;
-          bty = t;
-          bbpc = k;
+      if (b->fixed_bpc) cand.emplace_back(t, b->fixed_bpc);
+      else
+        for (int k : kBatchBpc) cand.emplace_back(t, k);
+    }
+    std::vector<float> best_ms(cand.size(), 1e30f);
+    for (int round = 0; round < 3; ++round) {
+      int built = 0;
+      for (size_t c = 0; c < cand.size(); ++c) {
+        if (cand[c].first != built) {
+          const int r = build_items(b, cand[c].first);
+          if (r != AUSM_OK) return r;
+          built = cand[c].first;
         }
+        best_ms[c] = std::min(best_ms[c], time_schedule(b, cand[c].first, cand[c].second));
       }
     }
+    size_t bi = 0;
+    for (size_t c = 1; c < cand.size(); ++c)
+      if (best_ms[c] < best_ms[bi]) bi = c;
+    const int bty = cand[bi].first, bbpc = cand[bi].second;
     b->tuned[cls] = {bty, bbpc};
     ty = bty;
     bpc = bbpc;
