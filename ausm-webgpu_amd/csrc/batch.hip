@@ -389,7 +389,7 @@ static float time_schedule(ausm_batch* b, int ty, int bpc) {
 
 static const int kBatchBpc[] = {1, 2, 4, 8};
 
-// choose (rows per wavefront, workgroups per CU) for the current mix: three
+// choose (rows per wavefront, workgroups per CU) for the current mix: two
 // rounds over every candidate, a candidate's time its best round (the
 // rounds interleave the candidates, so clock ramps and queue noise do not
 // favour the ones timed last)
@@ -406,15 +406,28 @@ static int choose_schedule(ausm_batch* b) {
     if (!bpc) bpc = it->second.second;
   }
   if (!ty || !bpc) {
+    // a workgroup count is a candidate only while the one below it still
+    // leaves some workgroup more than one item (beyond that the runs are
+    // no shorter, only more workgroups find nothing to do)
     std::vector<std::pair<int, int>> cand;
     for (int t : kBatchTiles) {
       if (b->fixed_ty && t != b->fixed_ty) continue;
-      if (b->fixed_bpc) cand.emplace_back(t, b->fixed_bpc);
-      else
-        for (int k : kBatchBpc) cand.emplace_back(t, k);
+      if (b->fixed_bpc) {
+        cand.emplace_back(t, b->fixed_bpc);
+        continue;
+      }
+      long long items = 0;
+      for (auto* s : b->table)
+        if (s) items += stage_blocks(s->d.M, s->d.N, t);
+      int prev = 0;
+      for (int k : kBatchBpc) {
+        if (prev && (long long)b->cus * prev >= items) break;
+        cand.emplace_back(t, k);
+        prev = k;
+      }
     }
     std::vector<float> best_ms(cand.size(), 1e30f);
-    for (int round = 0; round < 3; ++round) {
+    for (int round = 0; round < 2; ++round) {
       int built = 0;
       for (size_t c = 0; c < cand.size(); ++c) {
         if (cand[c].first != built) {

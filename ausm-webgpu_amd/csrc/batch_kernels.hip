@@ -89,11 +89,15 @@ __device__ __forceinline__ float batch_tile(int scheme, const DevParams& P, cons
 template <bool FAST, int TY, bool COMP, int STAGE>
 __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
   __shared__ FaceLds<float> s_face;  // one buffer for every inlined tile variant
-  const int n = *b.n_items;
-  const int g = xcd_remap(blockIdx.x, gridDim.x);
-  const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
-  const int lo = g * per;
-  const int hi = min(n, lo + per);
+  // workgroup g takes items [g n / B, (g+1) n / B): runs of floor or ceil
+  // of n / B spread over every g, so every XCD's share is within one run of
+  // n / 8 (fixed ceil(n / B) runs would leave the last workgroups -- the
+  // last XCDs' -- without items whenever B does not divide n)
+  const long long n = *b.n_items;
+  const long long nb = gridDim.x;
+  const long long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int lo = (int)(g * n / nb);
+  const int hi = (int)((g + 1) * n / nb);
   for (int it = lo; it < hi; ++it) {
     const BatchItem item = b.items[it];
     const SolveDev* S = b.solves + item.solve;
