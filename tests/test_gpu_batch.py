@@ -87,6 +87,28 @@ def test_batch_matches_single_grid_bitwise(math, sched, tile, dt_mode):
             assert np.array_equal(got[1:N + 1], ref[1:N + 1]), c
 
 
+@pytest.mark.parametrize("sched", [(1, 1), (2, 1), (4, 2)])
+def test_batch_uneven_runs_bitwise(sched):
+    """More items than workgroups, so that the persistent workgroups take
+    runs of unequal length (one row per wavefront and one workgroup per CU:
+    about 520 items on 256 workgroups; more rows or workgroups: runs of one
+    or two, or workgroups left without items).  Every solve still equals
+    its single-grid run bitwise."""
+    steps = 30
+    cases = CASES + [dict(M=480, N=200, code=2412, mach=0.7, aoa=2.0, scheme=ausm.AUSMUP)]
+    grids = [grid(c["M"], c["N"], c["code"]) for c in cases]
+    with ausm.Batch(max_solves=8, math="fast") as b:
+        b.set_schedule(*sched)
+        ids = [b.submit(g, ausm.freestream(c["mach"], c["aoa"]), scheme=c["scheme"],
+                        max_steps=steps) for c, g in zip(cases, grids)]
+        b.advance(steps, check_every=10)
+        for c, g, sid in zip(cases, grids, ids):
+            got = b.state(sid)
+            ref = single_run(c, g, steps, "fast", 2)
+            N = c["N"]
+            assert np.array_equal(got[1:N + 1], ref[1:N + 1]), (sched, c)
+
+
 def annulus(M, N, r0=0.5, r1=3.0):
     th = 2.0 * np.pi * np.arange(M) / M
     r = r0 * (r1 / r0) ** (np.arange(N + 1) / N)
