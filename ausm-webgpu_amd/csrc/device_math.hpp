@@ -281,15 +281,15 @@ __device__ __forceinline__ void muscl_fast(int limiter, float wl, float wr, floa
     // one reciprocal serves both sides: 1/den_L = den_R / (den_L den_R).
     // den >= d^2 >= eps^2 keeps the product normal; if it overflows the
     // reciprocal is 0 and both sides fall back to first order.
+    // The half difference dF/2 rides on the shared reciprocal:
+    // phi_L dF/2 = n_L (n_L + d) * (den_R * (dF/2) / (den_L den_R)).
     const float d = fmaf(dF, dF, eps);
     const float d2 = d * d;
     const float nL = fmaf(dL, dF, eps), nR = fmaf(dF, dR, eps);
     const float denL = fmaf(nL, nL, d2), denR = fmaf(nR, nR, d2);
-    const float r = __builtin_amdgcn_rcpf(denL * denR);
-    const float phiL = (nL * (nL + d)) * (denR * r);
-    const float phiR = (nR * (nR + d)) * (denL * r);
-    L = fmaf(phiL, hF, wl);
-    Rr = fmaf(-phiR, hF, wr);
+    const float rh = __builtin_amdgcn_rcpf(denL * denR) * hF;
+    L = fmaf(nL * (nL + d), denR * rh, wl);
+    Rr = fmaf(-(nR * (nR + d)), denL * rh, wr);
   }
 }
 
@@ -326,8 +326,9 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
   float mdot, P_i;
   if constexpr (SCHEME == SCHEME_AUSMUP) {
     const float M_inf2 = P.minf_coef * L.r;
-    const float h_t = 0.5f * (h_L + h_R);
-    const float aStar = __builtin_amdgcn_sqrtf(fmaxf(1e-6f, P.astar_coef * h_t));
+    // astar_coef * (h_L + h_R)/2, with the exact halving moved onto the
+    // (uniform) coefficient
+    const float aStar = __builtin_amdgcn_sqrtf(fmaxf(1e-6f, (0.5f * P.astar_coef) * (h_L + h_R)));
     const float as2 = aStar * aStar;
     // min(a*^2 / max(u_L, a*), a*^2 / max(-u_R, a*)) = a*^2 / max(u_L, -u_R, a*);
     // a_i and 1/a_i from one reciprocal
@@ -338,7 +339,7 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float M_L = u_L * ia;
     const float M_R = u_R * ia;
     const float a_i2 = a_i * a_i;
-    const float Mbar2 = fmaf(u_L, u_L, u_R * u_R) * (0.5f * ia * ia);
+    const float Mbar2 = 0.5f * fmaf(M_L, M_L, M_R * M_R);
     const float M_o2 = fminf(1.0f, fmaxf(Mbar2, M_inf2));
     const float f_a = M_o2 * (2.0f - M_o2);
     const float rho_i = 0.5f * (L.r + R.r);
@@ -356,7 +357,7 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
                        __builtin_amdgcn_rcpf(f_a * rho_i * a_i2);
       M_i -= kp;
     }
-    const float alpha4 = 0.75f * fmaf(5.0f * f_a, f_a, -4.0f);
+    const float alpha4 = fmaf(3.75f * f_a, f_a, -3.0f);  // 3/4 (5 f_a^2 - 4)
     // both sides evaluated, then selected
     const float p5sL = M2pL * fmaf(alpha4 * M_L, M2mL, 2.0f - M_L);
     const float p5sR = M2mR * fmaf(-alpha4 * M_R, M2pR, -2.0f - M_R);
