@@ -388,26 +388,18 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         // quad area (shaders/grid.js:140)
         const R area = R(0.5) * fabs((vx1r - vx0) * (vy1 - vy0r) - (vx1 - vx0r) * (vy1r - vy0));
 
-        // residual dQ/dt = -(F_R l_R - F_L l_L + G_U l_U - G_D l_D) / A  (ausm.js:434)
-        R res[4];
-        {
-          const R s0 = ((PR.f0 - PX.f0) + Gu.f0) - Gd.f0;
-          const R s1 = ((PR.f1 - PX.f1) + Gu.f1) - Gd.f1;
-          const R s2 = ((PR.f2 - PX.f2) + Gu.f2) - Gd.f2;
-          const R s3 = ((PR.f3 - PX.f3) + Gu.f3) - Gd.f3;
-          if constexpr (FAST) {
-            const float ia = __builtin_amdgcn_rcpf(area);
-            res[0] = -s0 * ia;
-            res[1] = -s1 * ia;
-            res[2] = -s2 * ia;
-            res[3] = -s3 * ia;
-          } else {
-            res[0] = (-s0) / area;
-            res[1] = (-s1) / area;
-            res[2] = (-s2) / area;
-            res[3] = (-s3) / area;
-          }
-        }
+        // residual dQ/dt = ns / A with ns = -(F_R l_R - F_L l_L + G_U l_U - G_D l_D)
+        // (ausm.js:434).  Exact math divides by A expression by expression as
+        // the reference does; fast math takes 1/A once, and its stage updates
+        // take the increment dt L = ns (dt / A) with fused multiply-adds.
+        const R ns[4] = {-(((PR.f0 - PX.f0) + Gu.f0) - Gd.f0), -(((PR.f1 - PX.f1) + Gu.f1) - Gd.f1),
+                         -(((PR.f2 - PX.f2) + Gu.f2) - Gd.f2), -(((PR.f3 - PX.f3) + Gu.f3) - Gd.f3)};
+        R ia = R(1);
+        if constexpr (FAST) ia = __builtin_amdgcn_rcpf(area);
+        auto res = [&](int k) -> R {
+          if constexpr (FAST) return ns[k] * ia;
+          else return ns[k] / area;
+        };
 
         if (owned) {
           R nq[4];
@@ -422,14 +414,26 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
             if constexpr (FAST) dt_c = fminf(P.maxdt, P.cfl * __builtin_amdgcn_rcpf(ws0));
             else dt_c = fmin(P.maxdt, P.cfl / ws0);
           }
+          R kdt = R(0);  // fast math: dt / A
+          if constexpr (FAST) kdt = dt_c * ia;
+          // the increment dt L(Q)
+          auto inc = [&](int k) -> R {
+            if constexpr (FAST) return ns[k] * kdt;
+            else return dt_c * res(k);
+          };
+          // c + dt L(Q)
+          auto add_inc = [&](int k, R c) -> R {
+            if constexpr (FAST) return fmaf(ns[k], kdt, c);
+            else return c + dt_c * res(k);
+          };
           if (stage == 0) {
-            for (int k = 0; k < 4; ++k) nq[k] = res[k];
+            for (int k = 0; k < 4; ++k) nq[k] = res(k);
           } else if constexpr (COMP) {
             R an[4];
             if (stage == 1) {
               // K1 = dt L(Q^n);  Q1 = hi + K1
               for (int k = 0; k < 4; ++k) {
-                an[k] = dt_c * res[k];
+                an[k] = inc(k);
                 nq[k] = cq[k] + an[k];
               }
             } else {
@@ -438,14 +442,15 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
               if (stage == 2) {
                 // K1 + K2;  Q2 = hi + (K1 + K2)/4
                 for (int k = 0; k < 4; ++k) {
-                  an[k] = ac[k] + dt_c * res[k];
-                  nq[k] = qh[k] + R(0.25) * an[k];
+                  an[k] = add_inc(k, ac[k]);
+                  if constexpr (FAST) nq[k] = fmaf(0.25f, an[k], qh[k]);
+                  else nq[k] = qh[k] + R(0.25) * an[k];
                 }
               } else {
                 // D = (K1 + K2 + 4 K3)/6 added to hi + lo by TwoSum
                 const R lw[4] = {lor[t].r, lor[t].mu, lor[t].mv, lor[t].e};
                 for (int k = 0; k < 4; ++k) {
-                  const R K = dt_c * res[k];
+                  const R K = inc(k);
                   R D;
                   if constexpr (FAST) D = fmaf(4.0f, K, ac[k]) * (1.0f / 6.0f);
                   else D = (ac[k] + R(4) * K) / R(6);
@@ -464,19 +469,22 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
             st_cons(dst + (size_t)r * (size_t)Pp, PS, bc, an[0], an[1], an[2], an[3]);
           } else if (stage == 1) {
             // Q1 = Qn + dt L(Qn)  (ausm.js:463)
-            for (int k = 0; k < 4; ++k) nq[k] = cq[k] + dt_c * res[k];
+            for (int k = 0; k < 4; ++k) nq[k] = add_inc(k, cq[k]);
           } else {
             const R qn[4] = {qnr[t].r, qnr[t].mu, qnr[t].mv, qnr[t].e};
             if (stage == 2) {
               // Q2 = 3/4 Qn + 1/4 (Q1 + dt L(Q1))  (ausm.js:494)
-              for (int k = 0; k < 4; ++k) nq[k] = R(0.75) * qn[k] + R(0.25) * (cq[k] + dt_c * res[k]);
+              if constexpr (FAST) {
+                for (int k = 0; k < 4; ++k) nq[k] = fmaf(0.25f, add_inc(k, cq[k]), 0.75f * qn[k]);
+              } else {
+                for (int k = 0; k < 4; ++k) nq[k] = R(0.75) * qn[k] + R(0.25) * add_inc(k, cq[k]);
+              }
             } else {
               // Qn+1 = (Qn + 2 (Q2 + dt L(Q2))) / 3  (ausm.js:525)
               if constexpr (FAST) {
-                for (int k = 0; k < 4; ++k)
-                  nq[k] = fmaf(2.0f, fmaf(dt_c, res[k], cq[k]), qn[k]) * (1.0f / 3.0f);
+                for (int k = 0; k < 4; ++k) nq[k] = fmaf(2.0f, add_inc(k, cq[k]), qn[k]) * (1.0f / 3.0f);
               } else {
-                for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * (cq[k] + dt_c * res[k])) / R(3);
+                for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * add_inc(k, cq[k])) / R(3);
               }
             }
           }
@@ -489,7 +497,7 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         }
 
         if (stage == 1 && a.res_out) {
-          const double rr = owned ? (double)res[0] * (double)res[0] : 0.0;
+          const double rr = owned ? (double)res(0) * (double)res(0) : 0.0;
           const double rsum = wave_sum(rr);
           if (lane == 0) a.res_out[r * a.tiles_x + tx] = rsum;
         }
