@@ -365,7 +365,7 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float P5L = supL ? stL : p5sL;
     const float P5R = supR ? stR : p5sR;
     const float ku = P.Ku * P5L * P5R * (L.r + R.r) * f_a * a_i * (u_R - u_L);
-    P_i = fmaf(P5L, L.p, P5R * R.p) - ku;
+    P_i = fmaf(P5L, L.p, fmaf(P5R, R.p, -ku));
     mdot = a_i * M_i * (M_i > 0.0f ? L.r : R.r);
   } else {
     const float a_L = __builtin_amdgcn_sqrtf(gamma * fmaxf(L.p, 1e-10f) * iclL);
@@ -379,8 +379,9 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
     const float X = (1.0f - Marc) * (1.0f - Marc);
     const bool supL = fabsf(M_L) >= 1.0f, supR = fabsf(M_R) >= 1.0f;
     const float qL = M_L + 1.0f, qR = M_R - 1.0f;
-    const float fpp = wsat(supL ? 0.5f * (1.0f + wsign(M_L)) : 0.25f * qL * qL * (2.0f - M_L));
-    const float fpm = wsat(supR ? 0.5f * (1.0f - wsign(M_R)) : 0.25f * qR * qR * (2.0f + M_R));
+    // supersonic side: |M| >= 1, so (1 +- sign(M))/2 is exactly the step M >< 0
+    const float fpp = wsat(supL ? (M_L > 0.0f ? 1.0f : 0.0f) : 0.25f * qL * qL * (2.0f - M_L));
+    const float fpm = wsat(supR ? (M_R < 0.0f ? 1.0f : 0.0f) : 0.25f * qR * qR * (2.0f + M_R));
     const float fsum = (fpp + fpm) - 1.0f;
     if constexpr (SCHEME == SCHEME_SLAU2) {
       P_i = 0.5f * fmaf(fpp - fpm, L.p - R.p, fmaf(Uarc * fsum, (L.r + R.r) * a_i, L.p + R.p));
