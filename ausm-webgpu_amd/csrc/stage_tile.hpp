@@ -235,6 +235,9 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     Cons raw[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
+      // slot 0 feeds only the bottom face, which the waves above the
+      // workgroup's bottom row block take from LDS (rb is wave-uniform)
+      if (k == 0 && rb != 0) continue;
       const int j = min(max(j0 - 2 + k, 0), N - 1);
       raw[k] = ld_cons(row_ptr(qin, j), PS, bc);
     }
@@ -304,9 +307,11 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     // difference serves up to three faces
     Prim w[NR], dd[NR];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) w[k] = prim_of(k);
+    for (int k = 0; k < NR; ++k)
+      if (k > 0 || rb == 0) w[k] = prim_of(k);
 #pragma unroll
-    for (int k = 1; k < NR; ++k) dd[k] = pdiff(w[k], w[k - 1]);
+    for (int k = 1; k < NR; ++k)
+      if (k > 1 || rb == 0) dd[k] = pdiff(w[k], w[k - 1]);
     // vertical face over slots k-2..k+1 with edge (ex, ey), times its length
     auto vface = [&](int k, R ex, R ey, R& len) -> Flux {
       R nx, ny;
