@@ -5,7 +5,9 @@
 // src/main.js:744-776, src/shaders/ausm.js:23-528):
 //   * a workgroup is four wavefronts stacked in j; each wavefront owns a
 //     strip of 60 cells around i (64 lanes, two halo lanes each side) and TY
-//     rows up j, and loads its whole window (rows j0-2 .. j0+TY+1) before any
+//     rows up j, and loads its whole window (rows j0-2 .. j0+TY+1; above the
+//     workgroup's bottom wavefront from j0-1, since row j0-2 only feeds the
+//     bottom face, which those wavefronts take from LDS) before any
 //     arithmetic;
 //   * each window row is converted to primitives once, and the differences
 //     of neighbouring rows that MUSCL works on are formed once and shared by
