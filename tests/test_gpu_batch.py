@@ -87,13 +87,14 @@ def test_batch_matches_single_grid_bitwise(math, sched, tile, dt_mode):
             assert np.array_equal(got[1:N + 1], ref[1:N + 1]), c
 
 
-@pytest.mark.parametrize("sched", [(1, 1), (2, 1), (4, 2)])
+@pytest.mark.parametrize("sched", [(1, 1), (2, 1), (4, 2), (0, 16)])
 def test_batch_uneven_runs_bitwise(sched):
     """More items than workgroups, so that the persistent workgroups take
     runs of unequal length (one row per wavefront and one workgroup per CU:
     about 520 items on 256 workgroups; more rows or workgroups: runs of one
-    or two, or workgroups left without items).  Every solve still equals
-    its single-grid run bitwise."""
+    or two, or workgroups left without items; 16 workgroups per CU with the
+    rows tuned: a workgroup count outside the tuner's own candidates).
+    Every solve still equals its single-grid run bitwise."""
     steps = 30
     cases = CASES + [dict(M=480, N=200, code=2412, mach=0.7, aoa=2.0, scheme=ausm.AUSMUP)]
     grids = [grid(c["M"], c["N"], c["code"]) for c in cases]
