@@ -411,12 +411,25 @@ __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, 
 // first / last interior row instead of being stored.
 // ---------------------------------------------------------------------------
 
+// division and square root of the ghost-row formulas: correctly rounded, or
+// (FAST, FP32 only) the hardware reciprocal and square root
+template <bool FAST, typename R>
+__device__ __forceinline__ R gdiv(R a, R b) {
+  if constexpr (FAST) return a * __builtin_amdgcn_rcpf(b);
+  else return a / b;
+}
+template <bool FAST, typename R>
+__device__ __forceinline__ R gsqrt(R a) {
+  if constexpr (FAST) return __builtin_amdgcn_sqrtf(a);
+  else return sqrt(a);
+}
+
 // slip wall: reflect the velocity of interior row 1 about the body-segment
 // normal (ausm.js:57-64).  (nx, ny) = rot90(normalize(v_i - v_{i+1})).
-template <typename R>
+template <typename R, bool FAST = false>
 __device__ __forceinline__ ConsT<R> wall_ghost(ConsT<R> I, R nx, R ny) {
   const R rho = I.r;
-  const R ux = I.mu / rho, uy = I.mv / rho;
+  const R ux = gdiv<FAST>(I.mu, rho), uy = gdiv<FAST>(I.mv, rho);
   const R k = R(2.0) * (nx * ux + ny * uy);
   const R gx = ux - k * nx, gy = uy - k * ny;
   ConsT<R> g;
@@ -429,12 +442,12 @@ __device__ __forceinline__ ConsT<R> wall_ghost(ConsT<R> I, R nx, R ny) {
 
 // far field from interior row N (ausm.js:65-117).  second = the outer ghost
 // row (N+2), which extrapolates linearly on outflow.
-template <typename R>
+template <typename R, bool FAST = false>
 __device__ __forceinline__ ConsT<R> far_ghost(const DevParamsT<R>& P, ConsT<R> I, R nx, R ny,
                                           bool second) {
   const R gamma = P.gamma;
   const R rho = I.r;
-  const R ux = I.mu / rho, uy = I.mv / rho;
+  const R ux = gdiv<FAST>(I.mu, rho), uy = gdiv<FAST>(I.mv, rho);
   const R bio = nx * P.vx + ny * P.vy;
   ConsT<R> g;
   if (bio < R(0.0)) {
@@ -444,28 +457,28 @@ __device__ __forceinline__ ConsT<R> far_ghost(const DevParamsT<R>& P, ConsT<R> I
     g.e = P.inState[3];
     return g;
   }
-  const R gm1inv = R(1.0) / (gamma - R(1.0));
+  const R gm1inv = gdiv<FAST>(R(1.0), gamma - R(1.0));
   const R u2 = ux * ux + uy * uy;
   const R Pr = (I.e - (R(0.5) * u2) * rho) * (gamma - R(1.0));
-  const R a2 = (gamma * Pr) / rho;
-  const R a2inf = (gamma * P.inP) / P.inRho;
-  const R M_int = u2 / a2;
-  const R M_inf2 = (P.vx * P.vx + P.vy * P.vy) / a2inf;
+  const R a2 = gdiv<FAST>(gamma * Pr, rho);
+  const R a2inf = gdiv<FAST>(gamma * P.inP, P.inRho);
+  const R M_int = gdiv<FAST>(u2, a2);
+  const R M_inf2 = gdiv<FAST>(P.vx * P.vx + P.vy * P.vy, a2inf);
   if (M_int < R(1.0) && M_inf2 < R(1.0)) {
     const R un_int = ux * nx + uy * ny;
     const R un_inf = bio;
-    const R a_int = sqrt(a2);
-    const R a_inf = sqrt(a2inf);
+    const R a_int = gsqrt<FAST>(a2);
+    const R a_inf = gsqrt<FAST>(a2inf);
     const R J_int = un_int + (R(2.0) * a_int) * gm1inv;
     const R J_ext = un_inf - (R(2.0) * a_inf) * gm1inv;
     const R un_g = R(0.5) * (J_int + J_ext);
     const R a_g = fmax(a_int * R(0.1), (R(0.25) * (J_int - J_ext)) * (gamma - R(1.0)));
-    const R X = fmax(R(1e-6), a_g / a_int);
+    const R X = fmax(R(1e-6), gdiv<FAST>(a_g, a_int));
     const R X2 = X * X;
     const R rho_g = fmax(R(1e-6), ((rho * X2) * X2) * X);
     const R dn = un_g - un_int;
     const R vgx = dn * nx + ux, vgy = dn * ny + uy;
-    const R rhoE_g = fmax(R(1e-6), ((rho_g * a_g) * a_g) / gamma) * gm1inv +
+    const R rhoE_g = fmax(R(1e-6), gdiv<FAST>((rho_g * a_g) * a_g, gamma)) * gm1inv +
                          (R(0.5) * rho_g) * (vgx * vgx + vgy * vgy);
     g.r = rho_g;
     g.mu = vgx * rho_g;
@@ -475,7 +488,7 @@ __device__ __forceinline__ ConsT<R> far_ghost(const DevParamsT<R>& P, ConsT<R> I
     g.r = I.r;
     g.mu = I.mu;
     g.mv = I.mv;
-    g.e = P.inP / (gamma - R(1.0)) + (R(0.5) * rho) * u2;
+    g.e = gdiv<FAST>(P.inP, gamma - R(1.0)) + (R(0.5) * rho) * u2;
   } else {
     g = I;
   }
@@ -486,6 +499,18 @@ __device__ __forceinline__ ConsT<R> far_ghost(const DevParamsT<R>& P, ConsT<R> I
     g.e = R(2.0) * g.e - I.e;
   }
   return g;
+}
+
+// the outer far-field ghost (row N+2) from the inner one (row N+1): the
+// same state on inflow, 2 g - I on outflow (ausm.js:114-116) -- what
+// far_ghost(..., true) returns, without evaluating the boundary state again
+template <typename R>
+__device__ __forceinline__ ConsT<R> far_ghost_outer(const DevParamsT<R>& P, ConsT<R> g, ConsT<R> I,
+                                                R nx, R ny) {
+  const R bio = nx * P.vx + ny * P.vy;
+  if (bio < R(0.0)) return g;
+  return ConsT<R>{R(2.0) * g.r - I.r, R(2.0) * g.mu - I.mu, R(2.0) * g.mv - I.mv,
+                  R(2.0) * g.e - I.e};
 }
 
 }  // namespace ausm

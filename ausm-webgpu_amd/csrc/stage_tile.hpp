@@ -277,21 +277,23 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
     };
     auto row_of = [&](int j) -> Cons { return ld_cons(row_ptr(qin, j), PS, bc); };
     Prim gW{}, gF1{}, gF2{};
+    // (fast math forms them with the hardware reciprocal and square root too)
     if (active && j0 < 2) {
       R nx, ny, len;
-      normal_exact(ld_row(a.vx, bc) - ld_row(a.vx, bcr), ld_row(a.vy, bc) - ld_row(a.vy, bcr), nx, ny,
-                   len);
-      gW = to_prim(wall_ghost(row_of(0), nx, ny));
+      face_normal<FAST>(ld_row(a.vx, bc) - ld_row(a.vx, bcr), ld_row(a.vy, bc) - ld_row(a.vy, bcr), nx,
+                        ny, len);
+      gW = to_prim(wall_ghost<R, FAST>(row_of(0), nx, ny));
     }
     if (active && j0 + TY + 2 > N) {
       const R* vxN = row_ptr(a.vx, N);
       const R* vyN = row_ptr(a.vy, N);
       R nx, ny, len;
-      normal_exact(ld_row(vxN, bc) - ld_row(vxN, bcr), ld_row(vyN, bc) - ld_row(vyN, bcr), nx, ny,
-                   len);
+      face_normal<FAST>(ld_row(vxN, bc) - ld_row(vxN, bcr), ld_row(vyN, bc) - ld_row(vyN, bcr), nx, ny,
+                        len);
       const Cons I = row_of(N - 1);
-      gF1 = to_prim(far_ghost(P, I, nx, ny, false));
-      gF2 = to_prim(far_ghost(P, I, nx, ny, true));
+      const Cons g1 = far_ghost<R, FAST>(P, I, nx, ny, false);
+      gF1 = to_prim(g1);
+      gF2 = to_prim(far_ghost_outer(P, g1, I, nx, ny));
     }
     // primitive state of window slot k (row j0-2+k)
     auto prim_of = [&](int k) -> Prim {
