@@ -200,10 +200,8 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
   const int own = min(kOwn, M - c0);
   int col = (c0 - 2 + lane) % M;
   if (col < 0) col += M;
-  const int colr = (col + 1 == M) ? 0 : col + 1;
   const bool owned = lane >= 2 && lane < 2 + own;
   const uint32_t bc = (uint32_t)col * (uint32_t)sizeof(R);    // this lane's column
-  const uint32_t bcr = (uint32_t)colr * (uint32_t)sizeof(R);  // and its right neighbour
   auto row_ptr = [&](const R* base, int j) { return base + (size_t)j * (size_t)Pp; };
   // row block of this wavefront among the workgroup's kWavesPerBlock: the
   // bottom block has one face more than the others, so the block order is
@@ -275,22 +273,50 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
       if constexpr (FAST) return to_prim_fast(q, P.gm1);
       else return to_prim_exact(q, P.gamma);
     };
-    auto row_of = [&](int j) -> Cons { return ld_cons(row_ptr(qin, j), PS, bc); };
     Prim gW{}, gF1{}, gF2{};
-    // (fast math forms them with the hardware reciprocal and square root too)
+    // (fast math forms them with the hardware reciprocal and square root
+    // too).  Their source rows are in the window already: row 0 is slot
+    // 2 - j0 and row N-1 slot N+1-j0; vertex rows 0 and N are vertex slots
+    // 0 and N-j0 when the window reaches them, else loaded.  A ghost row
+    // enters only the vertical faces of its own lane, so the right-hand
+    // vertex of the segment normal comes by a lane shift: lane 63, which
+    // gets none, is a halo lane whose faces and updates are not used.
     if (active && j0 < 2) {
+      R x0, y0;
+      if (j0 == 0) {
+        x0 = vxr[0];
+        y0 = vyr[0];
+      } else {
+        x0 = ld_row(a.vx, bc);
+        y0 = ld_row(a.vy, bc);
+      }
       R nx, ny, len;
-      face_normal<FAST>(ld_row(a.vx, bc) - ld_row(a.vx, bcr), ld_row(a.vy, bc) - ld_row(a.vy, bcr), nx,
-                        ny, len);
-      gW = to_prim(wall_ghost<R, FAST>(row_of(0), nx, ny));
+      face_normal<FAST>(x0 - from_right(x0), y0 - from_right(y0), nx, ny, len);
+      gW = to_prim(wall_ghost<R, FAST>(j0 == 0 ? raw[2] : raw[1], nx, ny));
     }
     if (active && j0 + TY + 2 > N) {
-      const R* vxN = row_ptr(a.vx, N);
-      const R* vyN = row_ptr(a.vy, N);
+      const int sN = N + 1 - j0;  // window slot of row N-1, in [2, TY+2]
+      Cons I = raw[2];
+#pragma unroll
+      for (int k = 3; k <= TY + 2; ++k)
+        if (k == sN) I = raw[k];
+      const int vN = N - j0;  // vertex slot of row N, in [1, TY+1]
+      R xN, yN;
+      if (vN <= TY) {
+        xN = vxr[1];
+        yN = vyr[1];
+#pragma unroll
+        for (int k = 2; k <= TY; ++k)
+          if (k == vN) {
+            xN = vxr[k];
+            yN = vyr[k];
+          }
+      } else {
+        xN = ld_row(row_ptr(a.vx, N), bc);
+        yN = ld_row(row_ptr(a.vy, N), bc);
+      }
       R nx, ny, len;
-      face_normal<FAST>(ld_row(vxN, bc) - ld_row(vxN, bcr), ld_row(vyN, bc) - ld_row(vyN, bcr), nx, ny,
-                        len);
-      const Cons I = row_of(N - 1);
+      face_normal<FAST>(xN - from_right(xN), yN - from_right(yN), nx, ny, len);
       const Cons g1 = far_ghost<R, FAST>(P, I, nx, ny, false);
       gF1 = to_prim(g1);
       gF2 = to_prim(far_ghost_outer(P, g1, I, nx, ny));
