@@ -86,10 +86,10 @@ def build_workload(ausm, M, N, grid_iters):
     s.set_flow(2.0, 0.0)
     s.set_scheme(ausm.AUSMUP, ausm.VANALBADA, 0.0, 0.75)
     s.set_cfl(0.5, 1.0, 1)
-    # two rows per wavefront tile: in a sweep over 1 / 2 / 4 / 8 rows at this
-    # size, 1 and 2 tie (26.2 / 26.5 us per step) and 4 / 8 are slower (36.1 /
-    # 61.9); fixed so that timing, trace and counter runs all measure the
-    # same kernel
+    # two rows per wavefront tile: in a sweep over 1 / 2 / 4 rows at this
+    # size, 2 is the fastest (19.9 us per step against 21.0 and 22.6; 8 rows
+    # were slower still in an earlier sweep); fixed so that timing, trace and
+    # counter runs all measure the same kernel
     s.set_tile(TILE_ROWS)
     return s, t_grid
 
