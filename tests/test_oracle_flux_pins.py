@@ -1,0 +1,506 @@
+"""Pins of the oracle's subsonic flux dissipation and far-field ghost.
+
+The consistency and supersonic pins leave the dissipative terms of the three
+schemes open: at Q_L = Q_R they cancel, and in the supersonic limit they
+vanish.  The cases below evaluate those terms at states chosen so that the
+reference's split functions take exact rational values, derived by hand
+from the reference text (src/shaders/ausm.js) and the papers it cites, and
+check the whole face flux the oracle returns:
+
+  AUSM+-up (Liou 2006, ausm.js:271-336): a_half from a* (eqs 28-30), the
+  4th-order Mach split M4 (:302-305), the 5th-order pressure split P5 with
+  alpha4 = 0.75 (-4 + 5 f_a^2) (:315-321), the K_p pressure diffusion of the
+  mass flux (eq 73, :313) and the K_u velocity diffusion of the pressure
+  (eq 75, :324), and M_inf^2 formed with rho_L (:273);
+
+  SLAU / SLAU2 (Kitamura & Shima 2011, 2013; ausm.js:345-391): the f_p split
+  (:361-370), chi = (1 - Mhat)^2 (:354-356), the SLAU pressure
+  1 + (1 - chi)(f+ + f- - 1) (:376), the SLAU2 pressure term
+  Ubar (f+ + f- - 1)(rho_L + rho_R) a_bar (:375), and the mass flux with the
+  chi/a_bar dp term and the reference's extra (1 - g) factor on the
+  convective part (:380-384);
+
+  the subsonic outflow ghost (ausm.js:84-106): it carries the interior
+  outgoing Riemann invariant J+, the free-stream incoming J-, the interior
+  tangential velocity and the interior entropy p/rho^gamma (rho_g from
+  (a_g/a)^5, the gamma = 1.4 isentrope); the second ghost row is 2 g - I
+  (:114-116); the supersonic-free-stream branch keeps the interior rho, rho u
+  and sets the ambient pressure (:107-109).
+
+A 1-D Sod shock tube stepped with the oracle's face flux and the
+reference's RK3 stage formulas must approach the exact Riemann solution
+(Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics", ch. 4),
+and every flux is mirror-antisymmetric: F(Q_L, Q_R, n) = -F(Q_R, Q_L, -n).
+
+Each case is chosen so that one of these plausible restatement mistakes
+fails it: a K_u term of the wrong sign or scale, a K_p term of the wrong
+sign, rho_inf for rho_L in M_inf^2, the SLAU2 Ubar term scaled, the SLAU
+(1 - chi) term scaled, the sign of the chi dp term, the (1 - g) factor
+dropped, and a wrong isentropic exponent in the ghost density.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from oracle_lib import (AUSMUP, NONE, SLAU, SLAU2, VANALBADA, Oracle, _ptr, circle_boundary,
+                        make_params64)
+
+GAMMA = 1.4
+TOL = 1e-12
+
+
+def cons(rho, u, v, p, gamma=GAMMA):
+    return np.array([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+def params(mach_x, K_p=0.0, K_u=0.75):
+    """FP64 parameters with gamma = 1.4, rho_inf = 1, p_inf = 1/1.4 (so
+    a_inf = 1) and the free stream mach_x along x; K_p and K_u as given
+    (both exactly representable in FP32)."""
+    return make_params64((1.0, 1.0 / GAMMA, mach_x, 0.0, GAMMA), K_p=K_p, K_u=K_u)
+
+
+def first_order(o, par, scheme, QL, QR, n):
+    """Face flux with no reconstruction (the stencil's outer states equal
+    the inner ones, uni.muscl = 0)."""
+    return o.face_flux(par, scheme, NONE, QL, QL, QR, QR, n)
+
+
+def check(F, expect):
+    expect = np.asarray(expect, dtype=np.float64)
+    scale = np.abs(expect).max()
+    assert np.abs(F - expect).max() <= TOL * scale, (F, expect)
+
+
+# ----------------------------------------------------------------------------
+# AUSM+-up
+# ----------------------------------------------------------------------------
+
+def test_ausmup_rest_pressure_jump_kp_diffusion():
+    """Both sides at rest, p_L = 1 > p_R = 0.6, rho_L = 1.25, rho_R = 0.8.
+    M_L = M_R = 0, so M4+(0) = 1/8, M4-(0) = -1/8 cancel and P5+(0) =
+    P5-(0) = 1/2: the pressure is the mean and the mass flux is the K_p
+    pressure diffusion alone,
+        M_half = -K_p (p_R - p_L) / (f_a rho_half a_half^2),
+    with a_half = a* = sqrt(h_bar / 3) (h = 3.5 p / rho at rest,
+    2 (gamma-1)/(gamma+1) = 1/3), Mbar^2 = 0 and f_a = M_o^2 (2 - M_o^2),
+    M_o^2 = M_inf^2 = |V_inf|^2 rho_L / (gamma p_inf) = 0.09 * 1.25.
+    Mass flows from high to low pressure, carrying h_L."""
+    o = Oracle(64)
+    rL, pL, rR, pR = 1.25, 1.0, 0.8, 0.6
+    QL, QR = cons(rL, 0, 0, pL), cons(rR, 0, 0, pR)
+    n = (0.6, 0.8)
+    hL, hR = 3.5 * pL / rL, 3.5 * pR / rR
+    a = math.sqrt((hL + hR) / 2 / 3)
+    Minf2 = 0.09 * rL
+    fa = Minf2 * (2 - Minf2)
+    for K_p in (0.25, 0.0):
+        par = params(0.3, K_p=K_p)
+        M_half = -K_p * (pR - pL) / (fa * 0.5 * (rL + rR) * a * a)
+        mdot = a * M_half * (rL if M_half > 0 else rR)
+        p_half = 0.5 * (pL + pR)
+        h_up = hL if mdot > 0 else hR
+        F = first_order(o, par, AUSMUP, QL, QR, n)
+        check(F, [mdot, n[0] * p_half, n[1] * p_half, mdot * h_up])
+    assert M_half == 0.0  # K_p = 0: no mass flux at rest
+
+
+def test_ausmup_symmetric_expansion_ku_diffusion():
+    """u_L = -1/2, u_R = +1/2 normal to the face (moving apart), rho = 1,
+    p = 23/28 on both sides, so h = 3.5 p + 1/8 = 3 and a* = 1 = a_half,
+    M_L = -1/2, M_R = 1/2, Mbar^2 = 1/4 (> M_inf^2 = 0.01), f_a = 7/16,
+    alpha4 = 0.75 (-4 + 5 (7/16)^2) = -2337/1024.
+      M2+(-1/2) = 1/16, M2-(-1/2) = -9/16, M2+(1/2) = 9/16, M2-(1/2) = -1/16
+      M4+(M_L) = (1/16)(1 - 9/8) = -1/128,  M4-(M_R) = (-1/16)(1 - 9/8) = 1/128
+    so the mass flux vanishes, and
+      P5+(M_L) = (1/16)(5/2 + (9/32) alpha4) = P5-(M_R) = 60887/524288 =: P5
+      p_half = 2 P5 p - K_u P5^2 (rho_L + rho_R) f_a a_half (u_R - u_L)
+             = 2 P5 p - (7/8) K_u P5^2:
+    the velocity diffusion lowers the pressure of an expansion."""
+    o = Oracle(64)
+    p = 23.0 / 28.0
+    P5 = 60887.0 / 524288.0
+    n = (0.0, 1.0)
+    QL, QR = cons(1.0, 0, -0.5, p), cons(1.0, 0, 0.5, p)
+    for K_u in (0.75, 0.25):
+        par = params(0.1, K_u=K_u)
+        p_half = 2 * P5 * p - 0.875 * K_u * P5 * P5
+        F = first_order(o, par, AUSMUP, QL, QR, n)
+        assert abs(F[0]) < 1e-15 and abs(F[3]) < 1e-14, F
+        check(F[1:3], [0.0, p_half])
+
+
+def test_ausmup_one_side_at_rest():
+    """u_L = 1/2 toward the face, u_R = 0, rho = 1, p_L = 1, p_R = 19/28
+    (h_L + h_R = 3.5 + 1/8 + 3.5 * 19/28 = 6, so a* = a_half = 1),
+    K_p = 1/4, K_u = 3/4, M_inf^2 = 0.01.
+      M_L = 1/2, M_R = 0, Mbar^2 = 1/8, f_a = (1/8)(15/8) = 15/64
+      M4+(1/2) = (9/16)(1 - 1/8) = 63/128,  M4-(0) = (-1/4)(1 - 1/2) = -1/8
+      K_p term: (1/4)(64/15)(7/8)(p_R - p_L) = (1/4)(64/15)(7/8)(-9/28) = -3/10
+      M_half = 47/128 + 3/10,  mdot = a_half M_half rho_L
+      alpha4 = 0.75 (-4 + 5 (15/64)^2) = -45777/16384
+      P5+(1/2) = (9/16)(3/2 - alpha4/32),  P5-(0) = (-1/4)(-2) = 1/2
+      K_u term: (3/4) P5+ (1/2) (2)(15/64)(1)(0 - 1/2) = -(45/512) P5+
+      p_half = P5+ p_L + p_R / 2 + (45/512) P5+
+    The convected state is the left one (mdot > 0): v_L = n/2,
+    h_L = 3.5 + 1/8."""
+    o = Oracle(64)
+    par = params(0.1, K_p=0.25, K_u=0.75)
+    pL, pR = 1.0, 19.0 / 28.0
+    n = (0.0, 1.0)
+    QL, QR = cons(1.0, 0, 0.5, pL), cons(1.0, 0, 0, pR)
+    M_half = 47.0 / 128.0 + 0.3
+    mdot = M_half
+    alpha4 = -45777.0 / 16384.0
+    P5p = (9.0 / 16.0) * (1.5 - alpha4 / 32.0)
+    p_half = P5p * pL + 0.5 * pR + (45.0 / 512.0) * P5p
+    F = first_order(o, par, AUSMUP, QL, QR, n)
+    check(F, [mdot, 0.0, mdot * 0.5 + p_half, mdot * 3.625])
+
+
+def test_ausmup_transonic_face():
+    """u_L = 5/4 > a* = 1 (rho = 1, p_L = 3/4, p_R = 83/112, so
+    h_L + h_R = 3.5 (3/4) + 25/32 + 3.5 (83/112) = 6), u_R = 0:
+      a^_L = a*^2 / max(u_L, a*) = 4/5 < a^_R = a* = 1, a_half = min = 4/5 (eq 28)
+      M_L = 25/16 >= 1 (supersonic: M4+ = M_L, P5+ = 1), M_R = 0
+      Mbar^2 = (25/16)/(2 (16/25)) > 1, so M_o^2 = 1, f_a = 1, alpha4 = 3/4
+      and the K_p term vanishes (max(1 - Mbar^2, 0) = 0)
+      M_half = 25/16 - 1/8 = 23/16,  mdot = (4/5)(23/16) = 23/20
+      p_half = p_L + p_R/2 - K_u (1)(1/2)(2)(1)(4/5)(0 - 5/4) = p_L + p_R/2 + 3/4
+    This face is the one case where the choice of a_half between the two
+    one-sided values shows: it scales the subsonic side's M4-."""
+    o = Oracle(64)
+    par = params(0.1, K_p=0.25, K_u=0.75)
+    pL, pR = 0.75, 83.0 / 112.0
+    QL, QR = cons(1.0, 0, 1.25, pL), cons(1.0, 0, 0, pR)
+    mdot = 23.0 / 20.0
+    p_half = pL + 0.5 * pR + 0.75
+    hL = 3.5 * pL + 25.0 / 32.0
+    F = first_order(o, par, AUSMUP, QL, QR, (0.0, 1.0))
+    check(F, [mdot, 0.0, mdot * 1.25 + p_half, mdot * hL])
+
+
+# ----------------------------------------------------------------------------
+# SLAU / SLAU2
+# ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("scheme", [SLAU, SLAU2])
+def test_slau_rest_pressure_jump(scheme):
+    """Both sides at rest: Ubar = 0, so Mhat = 0 and chi = 1; M = 0 gives
+    f+ = f- = 1/2 and g = 0.  Then p_half = (p_L + p_R)/2 in both versions,
+    and the mass flux is the pressure term alone,
+        mdot = -chi/(2 a_bar) (p_R - p_L) = (p_L - p_R) / (2 a_bar),
+    with a_bar the mean of the two sound speeds sqrt(gamma p / rho)."""
+    o = Oracle(64)
+    par = params(0.3)
+    rL, pL, rR, pR = 1.0, 1.0, 0.5, 0.3
+    n = (0.6, 0.8)
+    a_bar = 0.5 * (math.sqrt(GAMMA * pL / rL) + math.sqrt(GAMMA * pR / rR))
+    mdot = (pL - pR) / (2 * a_bar)
+    p_half = 0.5 * (pL + pR)
+    F = first_order(o, par, scheme, cons(rL, 0, 0, pL), cons(rR, 0, 0, pR), n)
+    check(F, [mdot, n[0] * p_half, n[1] * p_half, mdot * 3.5 * pL / rL])
+
+
+def _slau_pair(uL, uR):
+    """rho_L = 1, rho_R = 2, p = rho/1.4 (both sound speeds 1, a_bar = 1),
+    velocities normal to the face n = (0, 1)."""
+    rL, rR = 1.0, 2.0
+    pL, pR = rL / GAMMA, rR / GAMMA
+    return (rL, pL, cons(rL, 0, uL, pL)), (rR, pR, cons(rR, 0, uR, pR))
+
+
+@pytest.mark.parametrize("scheme", [SLAU, SLAU2])
+def test_slau_subsonic_one_side_at_rest(scheme):
+    """u_L = 1/2, u_R = 0, a_L = a_R = a_bar = 1, so M_L = 1/2, M_R = 0:
+      f+(1/2) = (1/2)(1/2)(3/2)^2 (2 - 1/2) = 27/32,  f-(0) = (1/2)(1/2)(1)(2) = 1/2
+      f+ + f- - 1 = 11/32 = f+ - f-
+      Ubar = sqrt((1/4 + 0)/2) = sqrt(1/8) = Mhat,  chi = (1 - Mhat)^2
+      SLAU2: p_half = (1/2)((11/32)(p_L - p_R) + (p_L + p_R) + Ubar (11/32)(rho_L + rho_R))
+      SLAU:  p_half = (1/2)((11/32)(p_L - p_R) + (1 + (1 - chi)(11/32))(p_L + p_R))
+      g = sat(-1/2) sat(0) = 0,  |Vbar_n| = (rho_L/2)/(rho_L + rho_R) = 1/6
+      mdot = (1/2)((1/2 + 1/6) + 2 (0 - 1/6) - chi (p_R - p_L)) = (1/2)(1/3 - chi/1.4)
+    mdot > 0, so the left state (v_L = n/2, h_L = 2.5 + 1/8) is convected."""
+    o = Oracle(64)
+    par = params(0.3)
+    (rL, pL, QL), (rR, pR, QR) = _slau_pair(0.5, 0.0)
+    U = math.sqrt(0.125)
+    chi = (1 - U) ** 2
+    s = 11.0 / 32.0
+    if scheme == SLAU2:
+        p_half = 0.5 * (s * (pL - pR) + (pL + pR) + U * s * (rL + rR))
+    else:
+        p_half = 0.5 * (s * (pL - pR) + (1 + (1 - chi) * s) * (pL + pR))
+    mdot = 0.5 * (1.0 / 3.0 - chi / GAMMA)
+    assert mdot > 0
+    F = first_order(o, par, scheme, QL, QR, (0.0, 1.0))
+    check(F, [mdot, 0.0, 0.5 * mdot + p_half, 2.625 * mdot])
+
+
+@pytest.mark.parametrize("scheme", [SLAU, SLAU2])
+def test_slau_subsonic_expansion_g_factor(scheme):
+    """u_L = -1/2, u_R = 1/4 (an expansion: g != 0), a_bar = 1:
+      f+(-1/2) = (1/2)(1/2)(1/2)^2 (5/2) = 5/32,  f-(1/4) = (1/2)(1/2)(3/4)^2 (9/4) = 81/256
+      f+ + f- - 1 = -135/256,  f+ - f- = -41/256
+      Ubar = sqrt((1/4 + 1/16)/2) = sqrt(5/32),  chi = (1 - Ubar)^2
+      g = sat(1/2) sat(1/4) = 1/8,  |Vbar_n| = (1/2 + 2/4)/3 = 1/3
+      |V|+ = (7/8)/3 + (1/8)(1/2) = 17/48,  |V|- = (7/8)/3 + (1/8)(1/4) = 31/96
+      convective part: (-1/2 + 17/48) + 2 (1/4 - 31/96) = -7/24
+      mdot = (1/2)((-7/24)(1 - 1/8) - chi (p_R - p_L))   [the reference's (1 - g)]
+    mdot < 0, so the right state (v_R = n/4, h_R = 2.5 + 1/32) is convected."""
+    o = Oracle(64)
+    par = params(0.3)
+    (rL, pL, QL), (rR, pR, QR) = _slau_pair(-0.5, 0.25)
+    U = math.sqrt(5.0 / 32.0)
+    chi = (1 - U) ** 2
+    fs, fd = -135.0 / 256.0, -41.0 / 256.0
+    if scheme == SLAU2:
+        p_half = 0.5 * (fd * (pL - pR) + (pL + pR) + U * fs * (rL + rR))
+    else:
+        p_half = 0.5 * (fd * (pL - pR) + (1 + (1 - chi) * fs) * (pL + pR))
+    mdot = 0.5 * ((-7.0 / 24.0) * (7.0 / 8.0) - chi * (pR - pL))
+    assert mdot < 0
+    F = first_order(o, par, scheme, QL, QR, (0.0, 1.0))
+    check(F, [mdot, 0.0, 0.25 * mdot + p_half, 2.53125 * mdot])
+
+
+# ----------------------------------------------------------------------------
+# symmetry of every flux
+# ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("scheme", [SLAU2, SLAU, AUSMUP])
+@pytest.mark.parametrize("limiter", [NONE, VANALBADA])
+def test_flux_mirror_antisymmetry(scheme, limiter):
+    """Reversing the stencil and the normal reverses the flux:
+    F(Q_L2, Q_L, Q_R, Q_R2; n) = -F(Q_R2, Q_R, Q_L, Q_L2; -n).  It holds for
+    all three schemes (M2+(-M) = -M2-(M), f+(-M) = f-(M), g and the K terms
+    are symmetric) as long as M_inf^2 = |V_inf|^2 rho_L/(gamma p_inf), the one
+    one-sided term, does not bind: the free stream here is M = 1e-3."""
+    o = Oracle(64)
+    par = params(1e-3, K_p=0.25, K_u=0.75)
+    rng = np.random.default_rng(31 + scheme * 3 + limiter)
+    for _ in range(100):
+        Qs = []
+        for _ in range(4):
+            rho, p = rng.uniform(0.5, 2.0), rng.uniform(0.4, 2.0)
+            a = math.sqrt(GAMMA * p / rho)
+            Qs.append(cons(rho, rng.uniform(-1.2, 1.2) * a, rng.uniform(-1.2, 1.2) * a, p))
+        th = rng.uniform(0, 2 * math.pi)
+        n = (math.cos(th), math.sin(th))
+        F = o.face_flux(par, scheme, limiter, *Qs, n)
+        G = o.face_flux(par, scheme, limiter, *Qs[::-1], (-n[0], -n[1]))
+        assert np.allclose(F, -G, rtol=1e-12, atol=1e-12 * np.abs(F).max()), (F, G)
+
+
+# ----------------------------------------------------------------------------
+# far-field ghost
+# ----------------------------------------------------------------------------
+
+def _annulus_state(o, par, M, N, seed, speed):
+    """Annulus grid (body r = 1, far field r = 3) and a state whose outermost
+    interior row N has a subsonic swirl plus outflow of the given speed."""
+    v = o.grid(circle_boundary(M, 1.0), circle_boundary(M, 3.0), N, 200)
+    rng = np.random.default_rng(seed)
+    st = o.init_state(par, M, N)
+    for i in range(M):
+        x, y = v[N, i]
+        r = math.hypot(x, y)
+        er, et = (x / r, y / r), (-y / r, x / r)
+        rho = rng.uniform(0.8, 1.2)
+        p = rng.uniform(0.6, 0.9)
+        ur, ut = speed * rng.uniform(0.5, 1.0), rng.uniform(-0.3, 0.3)
+        st[N, i] = cons(rho, ur * er[0] + ut * et[0], ur * er[1] + ut * et[1], p)
+    return v, st
+
+
+def _prim(Q):
+    rho = Q[0]
+    u, v = Q[1] / rho, Q[2] / rho
+    p = (Q[3] - 0.5 * rho * (u * u + v * v)) * (GAMMA - 1)
+    return rho, u, v, p
+
+
+def test_far_field_subsonic_outflow_ghost_invariants():
+    """Subsonic outflow faces (n . V_inf >= 0, interior and free stream
+    subsonic): the ghost keeps J+ = u_n + 2a/(gamma-1) of the interior,
+    takes J- = u_n,inf - 2 a_inf/(gamma-1) of the free stream, keeps the
+    interior tangential velocity and entropy p/rho^gamma; the outer ghost
+    row is 2 g - I.  Inflow faces get the free-stream state in both rows."""
+    o = Oracle(64)
+    M, N = 48, 6
+    par = params(0.5)
+    v, st = _annulus_state(o, par, M, N, seed=3, speed=0.3)
+    out = o.boundary(par, v, st)
+    Vinf = (float(par[3]), float(par[4]))
+    a_inf = math.sqrt(GAMMA * float(par[2]) / float(par[1]))
+    n_out = n_in = 0
+    for i in range(M):
+        dx, dy = v[N, i] - v[N, (i + 1) % M]
+        ln = math.hypot(dx, dy)
+        nx, ny = -dy / ln, dx / ln  # rot90(normalize(v_i - v_i+1)), outward
+        I, g, g2 = st[N, i], out[N + 1, i], out[N + 2, i]
+        if nx * Vinf[0] + ny * Vinf[1] < 0:
+            inflow = par[5:9].astype(np.float64)
+            assert np.array_equal(g, inflow) and np.array_equal(g2, inflow)
+            n_in += 1
+            continue
+        r, u, w, p = _prim(I)
+        rg, ug, wg, pg = _prim(g)
+        a, ag = math.sqrt(GAMMA * p / r), math.sqrt(GAMMA * pg / rg)
+        un, ung = u * nx + w * ny, ug * nx + wg * ny
+        assert ag > 0.1 * a  # the clamp is inactive
+        Jp, Jm = un + 5 * a, (nx * Vinf[0] + ny * Vinf[1]) - 5 * a_inf
+        assert math.isclose(ung + 5 * ag, Jp, rel_tol=1e-12, abs_tol=1e-12)
+        assert math.isclose(ung - 5 * ag, Jm, rel_tol=1e-12, abs_tol=1e-12)
+        assert math.isclose(-ug * ny + wg * nx, -u * ny + w * nx, abs_tol=1e-12)
+        assert math.isclose(pg / rg ** GAMMA, p / r ** GAMMA, rel_tol=1e-12)
+        assert np.allclose(g2, 2 * g - I, rtol=1e-14, atol=1e-14)
+        n_out += 1
+    assert n_out > 10 and n_in > 10
+
+
+def test_far_field_supersonic_free_stream_ghost():
+    """Outflow under a supersonic free stream: the ghost has the interior
+    rho and momentum and the ambient pressure p_inf (the reference applies it
+    whatever the interior Mach number); inflow faces get the free stream."""
+    o = Oracle(64)
+    M, N = 48, 6
+    par = params(2.0)
+    v, st = _annulus_state(o, par, M, N, seed=4, speed=1.5)
+    out = o.boundary(par, v, st)
+    for i in range(M):
+        dx, dy = v[N, i] - v[N, (i + 1) % M]
+        if -dy * float(par[3]) + dx * float(par[4]) < 0:
+            continue
+        I, g, g2 = st[N, i], out[N + 1, i], out[N + 2, i]
+        assert np.array_equal(g[:3], I[:3])
+        _, _, _, pg = _prim(g)
+        assert math.isclose(pg, float(par[2]), rel_tol=1e-12)
+        assert np.allclose(g2, 2 * g - I, rtol=1e-14, atol=1e-14)
+
+
+# ----------------------------------------------------------------------------
+# 1-D shock tube
+# ----------------------------------------------------------------------------
+
+def sod_exact(x, t, WL=(1.0, 0.0, 1.0), WR=(0.125, 0.0, 0.1), x0=0.5, g=GAMMA):
+    """Exact solution of the Riemann problem (Toro ch. 4): pressure in the
+    star region by Newton iteration on f_L(p) + f_R(p) + du = 0, then
+    sampling of the wave pattern at x/t."""
+    rL, uL, pL = WL
+    rR, uR, pR = WR
+    aL, aR = math.sqrt(g * pL / rL), math.sqrt(g * pR / rR)
+
+    def f(p, r, pk, a):
+        if p > pk:  # shock
+            A, B = 2 / ((g + 1) * r), (g - 1) / (g + 1) * pk
+            q = math.sqrt(A / (p + B))
+            return (p - pk) * q, q * (1 - (p - pk) / (2 * (B + p)))
+        e = (g - 1) / (2 * g)  # rarefaction
+        return 2 * a / (g - 1) * ((p / pk) ** e - 1), (p / pk) ** (-(g + 1) / (2 * g)) / (r * a)
+
+    p = 0.5 * (pL + pR)
+    for _ in range(50):
+        fL, dL = f(p, rL, pL, aL)
+        fR, dR = f(p, rR, pR, aR)
+        p = max(1e-8, p - (fL + fR + uR - uL) / (dL + dR))
+    fL, _ = f(p, rL, pL, aL)
+    fR, _ = f(p, rR, pR, aR)
+    us = 0.5 * (uL + uR) + 0.5 * (fR - fL)
+    rho = np.empty_like(x)
+    for k, xx in enumerate(x):
+        s = (xx - x0) / t
+        if s <= us:
+            if p > pL:
+                raise NotImplementedError
+            ash = aL * (p / pL) ** ((g - 1) / (2 * g))
+            if s <= uL - aL:
+                rho[k] = rL
+            elif s >= us - ash:
+                rho[k] = rL * (p / pL) ** (1 / g)
+            else:  # inside the fan
+                rho[k] = rL * (2 / (g + 1) + (g - 1) / ((g + 1) * aL) * (uL - s)) ** (2 / (g - 1))
+        else:
+            if p > pR:
+                S = uR + aR * math.sqrt((g + 1) / (2 * g) * p / pR + (g - 1) / (2 * g))
+                rho[k] = rR if s >= S else rR * (p / pR + (g - 1) / (g + 1)) / (
+                    (g - 1) / (g + 1) * p / pR + 1)
+            else:
+                raise NotImplementedError
+    return rho, p, us
+
+
+def sod_run(scheme, limiter, nc=200, t_end=0.2, cfl=0.5):
+    """Sod's tube on [0, 1] with nc cells: the oracle's face flux through
+    faces of normal (1, 0), two transmissive ghost cells per end, the
+    reference's TVD-RK3 stage formulas (ausm.js:463,494,525) and a global
+    CFL step cfl * dx / max(|u| + a)."""
+    o = Oracle(64)
+    par = params(0.0 + 1e-3)
+    fn = o.f("face_flux")
+    dx = 1.0 / nc
+    xc = (np.arange(nc) + 0.5) * dx
+    Q = np.array([cons(1.0, 0, 0, 1.0) if x < 0.5 else cons(0.125, 0, 0, 0.1) for x in xc])
+    parp = _ptr(np.ascontiguousarray(par))
+    F = np.zeros(4)
+    Fp = _ptr(F)
+    P = ctypes.c_void_p
+
+    def L(Qc):
+        ext = np.ascontiguousarray(np.vstack([Qc[:1], Qc[:1], Qc, Qc[-1:], Qc[-1:]]))
+        base, st = ext.ctypes.data, ext.strides[0]
+        flux = np.empty((nc + 1, 4))
+        for k in range(nc + 1):  # face k between cells k-1 and k (ext rows k+1, k+2)
+            fn(parp, scheme, limiter, P(base + k * st), P(base + (k + 1) * st),
+               P(base + (k + 2) * st), P(base + (k + 3) * st), 1.0, 0.0, Fp)
+            flux[k] = F
+        return -(flux[1:] - flux[:-1]) / dx
+
+    t = 0.0
+    while t < t_end - 1e-12:
+        r, u = Q[:, 0], Q[:, 1] / Q[:, 0]
+        p = (Q[:, 3] - 0.5 * r * u * u) * (GAMMA - 1)
+        dt = min(cfl * dx / (np.abs(u) + np.sqrt(GAMMA * p / r)).max(), t_end - t)
+        Q1 = Q + dt * L(Q)
+        Q2 = 0.75 * Q + 0.25 * (Q1 + dt * L(Q1))
+        Q = (Q + 2 * (Q2 + dt * L(Q2))) / 3
+        t += dt
+    return xc, Q
+
+
+# L1(rho) error bounds per scheme and reconstruction at 200 cells, t = 0.2
+SOD_L1 = {NONE: 0.02, VANALBADA: 0.008}
+
+
+@pytest.mark.parametrize("scheme", [SLAU2, SLAU, AUSMUP])
+@pytest.mark.parametrize("limiter", [NONE, VANALBADA])
+def test_sod_shock_tube(scheme, limiter):
+    """Sod's problem converges to the exact solution: L1(rho) within the
+    bound typical of a first-order / MUSCL upwind scheme at 200 cells, the
+    plateau between contact and shock at the exact star pressure, and the
+    shock within two cells of its exact position."""
+    xc, Q = sod_run(scheme, limiter)
+    rho_ex, p_star, u_star = sod_exact(xc, 0.2)
+    rho = Q[:, 0]
+    l1 = np.abs(rho - rho_ex).mean()
+    assert l1 < SOD_L1[limiter], l1
+    u = Q[:, 1] / rho
+    p = (Q[:, 3] - 0.5 * rho * u * u) * (GAMMA - 1)
+    # between the contact and the shock, clear of the shock's smeared profile
+    plateau = (xc > 0.5 + u_star * 0.2 + 0.02) & (xc < 0.5 + 1.7522 * 0.2 - 0.06)
+    assert np.abs(p[plateau] / p_star - 1).max() < 0.01, p[plateau]
+    assert np.abs(u[plateau] / u_star - 1).max() < 0.01
+    shock_ex = 0.5 + 1.7522 * 0.2
+    jump = xc[np.argmax(-np.diff(rho) * (xc[:-1] > 0.75))]
+    assert abs(jump - shock_ex) < 2.5 / 200, jump
+
+
+def test_sod_exact_solution_textbook_values():
+    """Toro's Table 4.2 star values for Sod's test: p* = 0.30313,
+    u* = 0.92745, rho*_L = 0.42632, rho*_R = 0.26557."""
+    x = np.array([0.5 + 0.8 * 0.2, 0.5 + 1.2 * 0.2])
+    rho, p, u = sod_exact(x, 0.2)
+    assert abs(p - 0.30313) < 5e-5 and abs(u - 0.92745) < 5e-5
+    assert abs(rho[0] - 0.42632) < 5e-5 and abs(rho[1] - 0.26557) < 5e-5
