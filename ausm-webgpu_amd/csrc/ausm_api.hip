@@ -245,7 +245,13 @@ hipError_t enqueue_step(ausm_ctx* c, int ty, bool fresh, bool refresh, bool norm
   return stage_launch(c, ty, 3, 2, 0, 0, false, refresh, false);
 }
 
+// captured step chunks kept per context (full chunks and remainders of
+// each tile shape and initial maxima count)
+constexpr size_t kMaxGraphs = 16;
+
 void drop_graphs(ausm_ctx* c) {
+  // a launched graph may still be running on the stream
+  if (!c->graphs.empty()) (void)hipStreamSynchronize(c->stream);
   for (auto& g : c->graphs) {
     hipGraphExecDestroy(g.exec);
     hipGraphDestroy(g.graph);
@@ -376,6 +382,9 @@ int ausm_sync(ausm_ctx* c) {
 
 // ---- grid -------------------------------------------------------------------
 static int finish_grid(ausm_ctx* c, const double* gx, const double* gy) {
+  // captured FP64 step graphs hold the previous grid's vertex pointers
+  // (g64[g_cur], which moves between the two ping-pong pairs)
+  drop_graphs(c);
   const size_t nv = (size_t)(c->N + 1) * c->P;
   CHECK_HIP(launch_f64_to_f32(gx, c->vx, nv, c->stream), "grid convert");
   CHECK_HIP(launch_f64_to_f32(gy, c->vy, nv, c->stream), "grid convert");
@@ -442,6 +451,7 @@ int ausm_grid_set_vertices(ausm_ctx* c, const float* xy) {
   CHECK_HIP(hipMemcpyAsync(c->g64[0], dx.data(), sizeof(double) * dx.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
   CHECK_HIP(hipMemcpyAsync(c->g64[1], dy.data(), sizeof(double) * dy.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
   CHECK_HIP(hipStreamSynchronize(c->stream), "vertex upload");
+  drop_graphs(c);  // FP64 graphs may hold the other vertex pair
   c->has_grid = true;
   c->has_state = false;
   c->g_cur = 0;
@@ -782,43 +792,56 @@ int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
   hipSetDevice(c->device);
   const int ty = current_rows(c);
   const int K = c->refresh;
-  // graph chunk: a multiple of the dt-refresh cadence, about 32 steps
+  // graph chunk: a multiple of the dt-refresh cadence, about 32 steps; a
+  // shorter remainder of whole refresh periods is a graph of its own, so
+  // only the steps of a partial period run as single launches
   const int G = K * std::max(1, 32 / K);
   int left = nsteps;
   while (left > 0) {
     const bool aligned = c->since_refresh == 0 && c->ws_fresh;
-    if (aligned && left >= G && c->use_graphs) {
+    if (aligned && left >= K && c->use_graphs) {
+      const int len = left >= G ? G : left / K * K;
       GraphEntry* ge = nullptr;
       for (auto& g : c->graphs)
-        if (g.ty == ty && g.steps == G && g.n_ws_in == c->n_ws_valid) ge = &g;
+        if (g.ty == ty && g.steps == len && g.n_ws_in == c->n_ws_valid) ge = &g;
       if (!ge) {
-        // capture G steps; within the chunk step s is fresh iff s % K == 0
+        if (c->graphs.size() >= kMaxGraphs) {
+          (void)hipStreamSynchronize(c->stream);
+          // oldest first: the full-length chunks are captured first and
+          // recaptured at once if they are ever needed again
+          hipGraphExecDestroy(c->graphs.front().exec);
+          hipGraphDestroy(c->graphs.front().graph);
+          c->graphs.erase(c->graphs.begin());
+        }
+        // capture len steps; within the chunk step s is fresh iff s % K == 0
         GraphEntry g{};
         g.ty = ty;
-        g.steps = G;
+        g.steps = len;
         g.n_ws_in = c->n_ws_valid;
         CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
         const int save_valid = c->n_ws_valid;
         hipError_t e = hipSuccess;
-        for (int s = 0; s < G && e == hipSuccess; ++s) {
+        for (int s = 0; s < len && e == hipSuccess; ++s) {
           const bool fresh = (s % K) == 0;
           const bool refresh = ((s + 1) % K) == 0;
-          e = enqueue_step(c, ty, fresh, refresh, s == G - 1);
+          e = enqueue_step(c, ty, fresh, refresh, s == len - 1);
           if (refresh) c->n_ws_valid = stage_blocks(c->M, c->N, ty);
         }
-        hipGraph_t graph;
+        hipGraph_t graph = nullptr;
         hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
         c->n_ws_valid = save_valid;
-        if (e != hipSuccess) return hip_fail(e, "graph capture (launch)");
-        if (e2 != hipSuccess) return hip_fail(e2, "graph capture");
+        if (e != hipSuccess || e2 != hipSuccess) {
+          if (graph) hipGraphDestroy(graph);
+          return e != hipSuccess ? hip_fail(e, "graph capture (launch)") : hip_fail(e2, "graph capture");
+        }
         g.graph = graph;
         CHECK_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0), "graph instantiate");
         c->graphs.push_back(g);
         ge = &c->graphs.back();
       }
       CHECK_HIP(hipGraphLaunch(ge->exec, c->stream), "graph launch");
-      c->steps += G;
-      left -= G;
+      c->steps += len;
+      left -= len;
       c->n_ws_valid = stage_blocks(c->M, c->N, ty);
       c->ws_fresh = true;
     } else {

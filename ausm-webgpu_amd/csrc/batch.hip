@@ -251,27 +251,10 @@ static int seed_ws(ausm_batch* b, Slot* s) {
 // stage-1 pass of one solve into its scratch Q1: residual partials of the
 // current state and maxws_cur; returns the FP64 density-residual norm
 static int solve_norm(ausm_batch* b, Slot* s, double* out) {
-  StageArgs a{};
-  a.qin = s->dev.q[0];
-  a.qn = s->dev.q[0];
-  a.qout = s->dev.q[1];
-  a.vx = s->dev.vx;
-  a.vy = s->dev.vy;
+  StageArgs a = batch_args(s->dev, 1, 0, 1);
   a.par = s->dpar;
-  a.ws_in = s->dev.ws_part;
-  a.ws_out = s->dev.ws_part;
-  a.res_out = s->dev.res_part;
-  a.maxws_cur = s->dev.maxws_cur;
+  a.fresh = 1;  // reduce the seeded maxima into maxws_cur
   a.n_ws_in = layout_of(s->d.M, s->d.N, b->comp).nws_max;
-  a.M = s->d.M;
-  a.N = s->d.N;
-  a.P = s->dev.P;
-  a.PS = s->dev.PS;
-  a.stage = 1;
-  a.limiter = s->d.limiter;
-  a.fresh = 1;
-  a.refresh = 0;
-  a.tiles_x = s->dev.tiles_x;
   a.local_dt = s->d.dt_mode == AUSM_DT_LOCAL;
   if (launch_stage(a, s->d.scheme, b->math, 2, b->stream) != hipSuccess ||
       launch_reduce_sum_f64(s->dev.res_part, s->d.N * s->dev.tiles_x, b->d_tmpd, b->stream) !=

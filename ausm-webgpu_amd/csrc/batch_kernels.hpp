@@ -39,6 +39,58 @@ struct BatchArgs {
   int norm;            // stage 1: write the residual-norm partials
 };
 
+// The stage arguments of one solve of the batch: the persistent kernel
+// builds them per item, and the host's initial-residual pass of a newly
+// admitted solve builds its stage-1 launch from the same function.  par
+// is left null: the batch kernel passes each solve's parameters by value.
+__host__ __device__ inline StageArgs batch_args(const SolveDev& S, int stage, int refresh,
+                                                int norm) {
+  StageArgs a{};
+  a.vx = S.vx;
+  a.vy = S.vy;
+  a.par = nullptr;
+  a.ws_in = S.ws_part;
+  a.ws_out = S.ws_part;
+  a.res_out = norm ? S.res_part : nullptr;
+  a.maxws_cur = S.maxws_cur;
+  a.n_ws_in = S.n_ws;
+  a.M = S.M;
+  a.N = S.N;
+  a.P = S.P;
+  a.PS = S.PS;
+  a.limiter = S.limiter;
+  a.tiles_x = S.tiles_x;
+  a.local_dt = S.local_dt;
+  a.acc = S.acc;
+  a.lo = S.lo;
+  a.fresh = 0;
+  a.refresh = refresh;
+  if (stage == 1) {
+    a.stage = 1;
+    a.qin = S.q[0];
+    a.qn = S.q[0];
+    a.qout = S.q[1];
+  } else if (stage == 2) {
+    a.stage = 2;
+    a.qin = S.q[1];
+    a.qn = S.q[0];
+    a.qout = S.q[2];
+  } else if (stage == 3) {
+    a.stage = 3;
+    a.qin = S.q[2];
+    a.qn = S.q[0];
+    a.qout = S.q[0];
+  } else {
+    // schedule tuning: stage-2 arithmetic from Q^n into the scratch Q2
+    a.stage = 2;
+    a.qin = S.q[0];
+    a.qn = S.q[0];
+    a.qout = S.q[2];
+    a.refresh = 0;
+  }
+  return a;
+}
+
 // comp: compensated-state stages (every solve of the batch)
 hipError_t launch_batch_stage(const BatchArgs& b, int fast, int comp, int ty, int blocks,
                               hipStream_t s);

@@ -52,6 +52,27 @@ using StageArgs64 = StageArgsT<double>;
 
 int stage_blocks(int M, int N, int ty);
 int stage_tiles_x(int M);
+
+// Host-side check of one stage launch: every pointer the stage dereferences
+// is set and the sizes agree with the layout the kernel assumes.  The
+// launchers refuse arguments that fail it (hipErrorInvalidValue) instead of
+// letting the kernel read through a null pointer.  comp: compensated-state
+// stages (which also use acc and lo).
+template <typename R>
+inline bool stage_args_ok(const StageArgsT<R>& a, bool comp) {
+  if (!a.qin || !a.qout || !a.vx || !a.vy || !a.par) return false;
+  if (a.stage < 0 || a.stage > 3) return false;
+  if (a.M < 4 || a.N < 4 || a.P < a.M || a.P % kWave != 0) return false;
+  if ((long long)a.PS < (long long)a.N * a.P || a.tiles_x != stage_tiles_x(a.M)) return false;
+  if (a.stage >= 1) {
+    if (!a.maxws_cur) return false;                                         // dt of the step
+    if (a.stage == 1 && a.fresh && (!a.ws_in || a.n_ws_in < 1)) return false;  // fresh max
+    if (a.stage >= 2 && !a.qn) return false;
+    if (a.stage == 3 && a.refresh && !a.ws_out) return false;
+    if (comp && (!a.acc || (a.stage == 3 && !a.lo))) return false;
+  }
+  return true;
+}
 hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s);
 // all-FP64 stage (exact arithmetic only); ty in {1, 2, 4}
 hipError_t launch_stage64(const StageArgs64& a, int scheme, int ty, hipStream_t s);

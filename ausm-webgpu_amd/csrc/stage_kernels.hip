@@ -118,6 +118,7 @@ static hipError_t launch_ty(const StageArgs& a, int ty, hipStream_t s) {
 }
 
 hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s) {
+  if (!stage_args_ok(a, false)) return hipErrorInvalidValue;
   if (fast) {
     switch (scheme) {
       case SCHEME_SLAU2: return launch_ty<SCHEME_SLAU2, true>(a, ty, s);
@@ -163,6 +164,7 @@ static hipError_t launch_ty_comp(const StageArgs& a, int ty, hipStream_t s) {
 }
 
 hipError_t launch_stage_comp(const StageArgs& a, int scheme, int fast, int ty, hipStream_t s) {
+  if (!stage_args_ok(a, true)) return hipErrorInvalidValue;
   switch (scheme * 2 + (fast ? 1 : 0)) {
     case SCHEME_SLAU2 * 2: return launch_ty_comp<SCHEME_SLAU2, false>(a, ty, s);
     case SCHEME_SLAU2 * 2 + 1: return launch_ty_comp<SCHEME_SLAU2, true>(a, ty, s);
@@ -201,6 +203,7 @@ static hipError_t launch_ty64(const StageArgs64& a, int ty, hipStream_t s) {
 }
 
 hipError_t launch_stage64(const StageArgs64& a, int scheme, int ty, hipStream_t s) {
+  if (!stage_args_ok(a, false)) return hipErrorInvalidValue;
   switch (scheme) {
     case SCHEME_SLAU2: return launch_ty64<SCHEME_SLAU2>(a, ty, s);
     case SCHEME_SLAU: return launch_ty64<SCHEME_SLAU>(a, ty, s);

@@ -185,7 +185,7 @@ class AusmSolver {
     let info = null;
     if (this.running) info = native.step(this.ctx, this.stepsPerFrame);
     this._lastTotal = now() - start;
-    this._ramp();
+    if (this.running) this._ramp();  // only frames that step ramp (main.js:742-784)
     return info;
   }
 
@@ -196,7 +196,7 @@ class AusmSolver {
     let info = null;
     if (this.running) info = await native.stepAsync(this.ctx, this.stepsPerFrame);
     this._lastTotal = now() - start;
-    this._ramp();
+    if (this.running) this._ramp();
     return info;
   }
 

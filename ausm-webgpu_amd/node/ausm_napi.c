@@ -37,10 +37,22 @@ static napi_value throw_ausm(napi_env env, int rc) {
     if (_rc != AUSM_OK) return throw_ausm((env), _rc); \
   } while (0)
 
+/* a context with the grid size it was created for: typed arrays crossing
+ * the boundary are checked against it (the library reads and writes
+ * arrays of exactly these sizes) */
+typedef struct {
+  ausm_ctx* c;
+  int M, N;
+} ctx_box;
+
 static void ctx_finalize(napi_env env, void* data, void* hint) {
   (void)env;
   (void)hint;
-  if (data) ausm_ctx_destroy((ausm_ctx*)data);
+  ctx_box* b = (ctx_box*)data;
+  if (b) {
+    ausm_ctx_destroy(b->c);
+    free(b);
+  }
 }
 
 /* ---- argument helpers ---------------------------------------------------- */
@@ -60,13 +72,38 @@ static double num(napi_env env, napi_value v) {
   return d;
 }
 
-static ausm_ctx* ctx_of(napi_env env, napi_value v) {
+static ctx_box* box_of(napi_env env, napi_value v) {
   void* p = NULL;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
     napi_throw_type_error(env, NULL, "expected an ausm context");
     return NULL;
   }
-  return (ausm_ctx*)p;
+  return (ctx_box*)p;
+}
+
+static ausm_ctx* ctx_of(napi_env env, napi_value v) {
+  ctx_box* b = box_of(env, v);
+  return b ? b->c : NULL;
+}
+
+/* RangeError unless a typed array holds exactly `want` elements */
+static int check_len(napi_env env, size_t got, size_t want, const char* what) {
+  if (got == want) return 1;
+  char msg[200];
+  snprintf(msg, sizeof msg, "%s: expected %zu elements, got %zu", what, want, got);
+  napi_throw_range_error(env, NULL, msg);
+  return 0;
+}
+
+/* RangeError unless the (M, N) a caller passes are the context's own */
+static int check_dims(napi_env env, const ctx_box* b, napi_value m, napi_value n) {
+  const int M = (int)num(env, m), N = n ? (int)num(env, n) : b->N;
+  if (M == b->M && N == b->N) return 1;
+  char msg[200];
+  snprintf(msg, sizeof msg, "grid size %d x %d does not match the context's %d x %d", M, N, b->M,
+           b->N);
+  napi_throw_range_error(env, NULL, msg);
+  return 0;
 }
 
 /* pointer and element count of a typed array of the given type */
@@ -104,9 +141,22 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   napi_value a[3];
   if (!get_args(env, info, 3, a)) return NULL;
   ausm_ctx* c = NULL;
-  AUSM_CALL(env, ausm_ctx_create((int)num(env, a[2]), (int)num(env, a[0]), (int)num(env, a[1]), &c));
+  const int M = (int)num(env, a[0]), N = (int)num(env, a[1]);
+  AUSM_CALL(env, ausm_ctx_create((int)num(env, a[2]), M, N, &c));
+  ctx_box* b = (ctx_box*)malloc(sizeof(ctx_box));
+  if (!b) {
+    ausm_ctx_destroy(c);
+    return throw_ausm(env, AUSM_ERR_NOMEM);
+  }
+  b->c = c;
+  b->M = M;
+  b->N = N;
   napi_value ext;
-  NAPI_CALL(env, napi_create_external(env, c, ctx_finalize, NULL, &ext));
+  if (napi_create_external(env, b, ctx_finalize, NULL, &ext) != napi_ok) {
+    ctx_finalize(env, b, NULL);
+    napi_throw_error(env, NULL, "N-API call failed: napi_create_external");
+    return NULL;
+  }
   return ext;
 }
 
@@ -118,6 +168,7 @@ static napi_value js_naca4(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_naca4(num(env, a[0]), num(env, a[1]), num(env, a[2]), num(env, a[3]),
                                      num(env, a[4]), num(env, a[5]), (int)num(env, a[6]), M, (double*)d));
   return out;
@@ -130,6 +181,7 @@ static napi_value js_circle(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_circle(num(env, a[0]), M, (double*)d));
   return out;
 }
@@ -141,6 +193,7 @@ static napi_value js_rect(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_rect(num(env, a[0]), M, (double*)d));
   return out;
 }
@@ -158,6 +211,7 @@ static napi_value js_cst(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_cst(au, al, (int)nu, num(env, a[2]), num(env, a[3]), num(env, a[4]),
                                    num(env, a[5]), (int)num(env, a[6]), M, (double*)d));
   return out;
@@ -176,6 +230,7 @@ static napi_value js_bezier(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_bezier(cu, (int)(nu / 2), cl, (int)(nl / 2), num(env, a[2]),
                                       num(env, a[3]), num(env, a[4]), (int)num(env, a[5]), M,
                                       (double*)d));
@@ -189,6 +244,7 @@ static napi_value js_sears_haack(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_sears_haack(num(env, a[0]), num(env, a[1]), num(env, a[2]), M, (double*)d));
   return out;
 }
@@ -200,6 +256,7 @@ static napi_value js_haack(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_haack_projectile(num(env, a[0]), num(env, a[1]), num(env, a[2]),
                                                 num(env, a[3]), num(env, a[4]), num(env, a[5]),
                                                 num(env, a[6]), M, (double*)d));
@@ -213,6 +270,7 @@ static napi_value js_polygon(napi_env env, napi_callback_info info) {
   if (M < 4) return throw_ausm(env, AUSM_ERR_ARG);
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) return NULL;
   AUSM_CALL(env, ausm_boundary_polygon(num(env, a[0]), (int)num(env, a[1]), num(env, a[2]),
                                        num(env, a[3]), num(env, a[4]), M, (double*)d));
   return out;
@@ -238,6 +296,10 @@ static napi_value js_dat(napi_env env, napi_callback_info info) {
   }
   void* d;
   napi_value out = new_typed(env, napi_float64_array, 2 * (size_t)M, &d);
+  if (!out) {
+    free(text);
+    return NULL;
+  }
   const int rc = ausm_boundary_dat(text, num(env, a[1]), num(env, a[2]), M, (double*)d, NULL);
   free(text);
   if (rc != AUSM_OK) return throw_ausm(env, rc);
@@ -248,11 +310,14 @@ static napi_value js_dat(napi_env env, napi_callback_info info) {
 static napi_value js_grid_from_boundary(napi_env env, napi_callback_info info) {
   napi_value a[6];
   if (!get_args(env, info, 6, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
+  ctx_box* b = box_of(env, a[0]);
+  if (!b) return NULL;
   size_t nb, no;
   double* body = (double*)typed(env, a[1], napi_float64_array, &nb);
+  if (!body || !check_len(env, nb, 2 * (size_t)b->M, "body points")) return NULL;
   double* outer = (double*)typed(env, a[2], napi_float64_array, &no);
-  if (!c || !body || !outer) return NULL;
+  if (!outer || !check_len(env, no, 2 * (size_t)b->M, "far-field points")) return NULL;
+  ausm_ctx* c = b->c;
   AUSM_CALL(env, ausm_grid_from_boundary(c, body, outer, (int)num(env, a[3]), (int)num(env, a[4]),
                                          num(env, a[5])));
   return NULL;
@@ -261,23 +326,25 @@ static napi_value js_grid_from_boundary(napi_env env, napi_callback_info info) {
 static napi_value js_set_vertices(napi_env env, napi_callback_info info) {
   napi_value a[2];
   if (!get_args(env, info, 2, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
+  ctx_box* b = box_of(env, a[0]);
+  if (!b) return NULL;
   size_t n;
   float* v = (float*)typed(env, a[1], napi_float32_array, &n);
-  if (!c || !v) return NULL;
-  AUSM_CALL(env, ausm_grid_set_vertices(c, v));
+  if (!v || !check_len(env, n, 2 * (size_t)b->M * ((size_t)b->N + 1), "vertices")) return NULL;
+  AUSM_CALL(env, ausm_grid_set_vertices(b->c, v));
   return NULL;
 }
 
 static napi_value js_get_vertices(napi_env env, napi_callback_info info) {
   napi_value a[3];
   if (!get_args(env, info, 3, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
-  if (!c) return NULL;
-  const size_t n = (size_t)num(env, a[1]) * ((size_t)num(env, a[2]) + 1) * 2;
+  ctx_box* b = box_of(env, a[0]);
+  if (!b || !check_dims(env, b, a[1], a[2])) return NULL;
+  const size_t n = (size_t)b->M * ((size_t)b->N + 1) * 2;
   void* d;
   napi_value out = new_typed(env, napi_float32_array, n, &d);
-  AUSM_CALL(env, ausm_grid_get_vertices(c, (float*)d));
+  if (!out) return NULL;
+  AUSM_CALL(env, ausm_grid_get_vertices(b->c, (float*)d));
   return out;
 }
 
@@ -333,23 +400,25 @@ static napi_value js_init_state(napi_env env, napi_callback_info info) {
 static napi_value js_set_state(napi_env env, napi_callback_info info) {
   napi_value a[2];
   if (!get_args(env, info, 2, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
+  ctx_box* b = box_of(env, a[0]);
+  if (!b) return NULL;
   size_t n;
   float* q = (float*)typed(env, a[1], napi_float32_array, &n);
-  if (!c || !q) return NULL;
-  AUSM_CALL(env, ausm_set_state(c, q));
+  if (!q || !check_len(env, n, 4 * (size_t)b->M * ((size_t)b->N + 3), "state")) return NULL;
+  AUSM_CALL(env, ausm_set_state(b->c, q));
   return NULL;
 }
 
 static napi_value js_get_state(napi_env env, napi_callback_info info) {
   napi_value a[3];
   if (!get_args(env, info, 3, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
-  if (!c) return NULL;
-  const size_t n = (size_t)num(env, a[1]) * ((size_t)num(env, a[2]) + 3) * 4;
+  ctx_box* b = box_of(env, a[0]);
+  if (!b || !check_dims(env, b, a[1], a[2])) return NULL;
+  const size_t n = (size_t)b->M * ((size_t)b->N + 3) * 4;
   void* d;
   napi_value out = new_typed(env, napi_float32_array, n, &d);
-  AUSM_CALL(env, ausm_get_state(c, (float*)d));
+  if (!out) return NULL;
+  AUSM_CALL(env, ausm_get_state(b->c, (float*)d));
   return out;
 }
 
@@ -427,13 +496,13 @@ static napi_value js_step_async(napi_env env, napi_callback_info info) {
 static napi_value js_forces(napi_env env, napi_callback_info info) {
   napi_value a[3];
   if (!get_args(env, info, 3, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
-  if (!c) return NULL;
-  const size_t M = (size_t)num(env, a[2]);
+  ctx_box* b = box_of(env, a[0]);
+  if (!b || !check_dims(env, b, a[2], NULL)) return NULL;
   double out[4];
   void* cp;
-  napi_value cpa = new_typed(env, napi_float64_array, M, &cp);
-  AUSM_CALL(env, ausm_forces(c, num(env, a[1]), out, (double*)cp));
+  napi_value cpa = new_typed(env, napi_float64_array, (size_t)b->M, &cp);
+  if (!cpa) return NULL;
+  AUSM_CALL(env, ausm_forces(b->c, num(env, a[1]), out, (double*)cp));
   napi_value o;
   napi_create_object(env, &o);
   set_num(env, o, "fx", out[0]);
@@ -447,12 +516,13 @@ static napi_value js_forces(napi_env env, napi_callback_info info) {
 static napi_value js_residual(napi_env env, napi_callback_info info) {
   napi_value a[3];
   if (!get_args(env, info, 3, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
-  if (!c) return NULL;
-  const size_t n = (size_t)num(env, a[1]) * (size_t)num(env, a[2]) * 4;
+  ctx_box* b = box_of(env, a[0]);
+  if (!b || !check_dims(env, b, a[1], a[2])) return NULL;
+  const size_t n = (size_t)b->M * (size_t)b->N * 4;
   void* d;
   napi_value out = new_typed(env, napi_float32_array, n, &d);
-  AUSM_CALL(env, ausm_eval_residual(c, (float*)d));
+  if (!out) return NULL;
+  AUSM_CALL(env, ausm_eval_residual(b->c, (float*)d));
   return out;
 }
 
@@ -478,13 +548,14 @@ static napi_value js_set_dt_mode(napi_env env, napi_callback_info info) {
 static napi_value js_derived_field(napi_env env, napi_callback_info info) {
   napi_value a[4];
   if (!get_args(env, info, 4, a)) return NULL;
-  ausm_ctx* c = ctx_of(env, a[0]);
-  if (!c) return NULL;
-  const size_t n = (size_t)num(env, a[2]) * (size_t)num(env, a[3]);
+  ctx_box* b = box_of(env, a[0]);
+  if (!b || !check_dims(env, b, a[2], a[3])) return NULL;
+  const size_t n = (size_t)b->M * (size_t)b->N;
   void* d;
   napi_value arr = new_typed(env, napi_float32_array, n, &d);
+  if (!arr) return NULL;
   float fs = 0.0f;
-  AUSM_CALL(env, ausm_derived_field(c, (int)num(env, a[1]), (float*)d, &fs));
+  AUSM_CALL(env, ausm_derived_field(b->c, (int)num(env, a[1]), (float*)d, &fs));
   napi_value obj;
   NAPI_CALL(env, napi_create_object(env, &obj));
   napi_set_named_property(env, obj, "field", arr);

@@ -204,3 +204,39 @@ def test_fp64_local_dt_bitwise():
         ref.step(chunk)
         assert np.array_equal(s.state()[1:N + 1], ref.state[1:N + 1]), chunk
     s.close()
+
+
+def test_fp64_graphs_follow_a_new_grid():
+    """A grid change on a context that has stepped through captured FP64
+    graphs: the FP64 solve reads the grid solver's vertex pair, which an odd
+    number of Jacobi sweeps leaves in the other ping-pong buffer, so graphs
+    captured on the old grid must not be replayed.  Stepping after the
+    regeneration is bit-for-bit a fresh context's run on the new grid."""
+    M, N = 64, 24
+    m, p, t = naca4_digits(2412)
+    body = naca4_boundary(M, m, p, t, size=0.5, x_offset=0.0, expo=1.4, spacing=1)
+    outer = circle_boundary(M, 10.0)
+    fs = ausm.freestream(0.7, 2.0)
+
+    def configured(iters):
+        s = ausm.Solver(M, N)
+        s.update_grid_boundaries(body, outer)
+        s.prepare_grid(iters=iters)
+        s.set_freestream(*fs)
+        s.set_scheme(SLAU2, VANALBADA, 0.0, 0.75)
+        s.set_cfl(0.5, 1.0, 1)
+        s.set_precision("fp64")
+        s.prepare_state(freestream=True)
+        return s
+
+    a = configured(40)   # even sweeps: the grid ends in the first pair
+    a.step(64)           # two captured 32-step chunks
+    a.prepare_grid(iters=41)  # odd: the second pair
+    a.prepare_state(freestream=True)
+    a.step(64)
+    b = configured(41)
+    b.step(64)
+    assert np.array_equal(a.vertices(fp64=True), b.vertices(fp64=True))
+    assert np.array_equal(a.state64(), b.state64())
+    a.close()
+    b.close()
