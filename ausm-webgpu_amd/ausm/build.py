@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libausm.so")
 BUILD_DIR = os.path.join(PKG, "_build")
 
 SOURCES = ["stage_kernels.hip", "batch_kernels.hip", "aux_kernels.hip", "ausm_api.hip",
-           "batch.hip", "geometry.cpp"]
+           "batch.hip", "noalloc.hip", "geometry.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AUSM_ARCH", "gfx950")

@@ -56,6 +56,17 @@ class SolveDesc(ctypes.Structure):
                 ("dt_mode", ctypes.c_int)]
 
 
+class StepDesc(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int),
+                ("rho", ctypes.c_double), ("p", ctypes.c_double), ("vx", ctypes.c_double),
+                ("vy", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("scheme", ctypes.c_int), ("limiter", ctypes.c_int),
+                ("K_p", ctypes.c_float), ("K_u", ctypes.c_float),
+                ("cfl", ctypes.c_float), ("maxdt", ctypes.c_float),
+                ("steps_per_dt_refresh", ctypes.c_int), ("math", ctypes.c_int),
+                ("dt_mode", ctypes.c_int), ("rows_per_wave", ctypes.c_int)]
+
+
 class SolveStatus(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int), ("steps", ctypes.c_longlong),
                 ("res_l2", ctypes.c_double), ("res_l2_0", ctypes.c_double),
@@ -107,6 +118,8 @@ SIGNATURES = {
     "ausm_get_state64": (_i, [_vp, _vp]),
     "ausm_eval_residual64": (_i, [_vp, _vp]),
     "ausm_profile_stages": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float)]),
+    "ausm_step_noalloc": (_i, [_i, ctypes.POINTER(StepDesc), _vp, _vp, _i, _vp,
+                               ctypes.POINTER(StepInfo)]),
     "ausm_batch_create": (_i, [_i, _i, ctypes.c_size_t, _pp]),
     "ausm_batch_destroy": (_i, [_vp]),
     "ausm_batch_stream": (_i, [_vp, _pp]),

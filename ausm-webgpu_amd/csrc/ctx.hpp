@@ -61,6 +61,10 @@ struct ausm_ctx {
 namespace ausm {
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
+size_t round_up(size_t x, size_t a);
+// the FP32 parameter block of a free stream (inState formed in double,
+// stored as FP32, as the reference's host does)
+void make_freestream(DevParams& p, double rho, double pr, double vx, double vy, double gamma);
 StageArgs stage_args(ausm_ctx* c, int ty, int stage);
 StageArgs64 stage_args64(ausm_ctx* c, int ty, int stage);
 }  // namespace ausm

@@ -246,6 +246,38 @@ int ausm_eval_residual64(ausm_ctx* ctx, double* res);
  * buffers). */
 int ausm_profile_stages(ausm_ctx* ctx, int reps, float* ms3);
 
+/* ---- allocation-free step ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc: nsteps TVD-RK3 steps of a caller-owned solution,
+ * the same kernels and launch sequence as ausm_step on a context given the
+ * same vertices, state and settings (bit-for-bit the same state), with no
+ * context and no allocation.  vertices ((N+1) x M x 2) and state
+ * ((N+3) x M x 4, updated in place, ghost rows rewritten) are DEVICE
+ * pointers in the reference's layouts on `device`; stream is a hipStream_t
+ * of that device (NULL: the null stream).  Every intermediate buffer comes
+ * from one fixed device array of AUSM_NOALLOC_SCRATCH_MB MiB per GPU, held
+ * until the process ends: grids needing more (about 1.2 million cells) are
+ * AUSM_ERR_ARG.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU race.  Plain FP32 (the reference's arithmetic) only;
+ * the first step takes its dt from the input state, later ones refresh it
+ * every steps_per_dt_refresh steps.  info (optional) is read back with a
+ * stream synchronisation; with info NULL the call only enqueues work. */
+#define AUSM_NOALLOC_SCRATCH_MB 64
+typedef struct ausm_step_desc {
+  int M, N;                   /* cells around / radial */
+  double rho, p, vx, vy, gamma;  /* free stream, as ausm_set_freestream */
+  int scheme, limiter;        /* as ausm_set_scheme */
+  float K_p, K_u;
+  float cfl, maxdt;           /* as ausm_set_cfl */
+  int steps_per_dt_refresh;
+  int math;                   /* AUSM_MATH_* */
+  int dt_mode;                /* AUSM_DT_* */
+  int rows_per_wave;          /* stage-kernel tile rows: 1, 2, 4, 8 (0 = 2) */
+} ausm_step_desc;
+int ausm_step_noalloc(int device, const ausm_step_desc* desc, const float* vertices, float* state,
+                      int nsteps, void* stream, ausm_step_info* info);
+
 /* ---- batched solves and continuous-batching service ---------------------
  * No counterpart in the reference (it runs one grid).  A batch context
  * holds many independent solves of different sizes resident in HBM and
