@@ -9,6 +9,10 @@ van Albada), TVD-RK3, CFL 0.5, 1000 steps.
 Bar: the exact-math GPU run is bit-for-bit the ref32 run of the same inputs
 (the restated WGSL), and the FP32 results (Cl, Cd, the state) stay within
 stated bounds of the all-FP64 run of the same 1000 steps on the same grid.
+
+Also at full size: the reference README's literal 510 x 381 grid, and the
+single-grid headline configuration (512 x 384, Mach 2, AUSM+-up) run 2000
+steps into the developed bow shock, bit-for-bit ref32 in state and residual.
 """
 import numpy as np
 import pytest
@@ -97,33 +101,78 @@ def test_literal_510x381_supersonic():
     s.close()
 
 
-def test_headline_fast_math_tracks_exact_over_2000_steps():
-    """The single-grid headline configuration (512 x 384 NACA 2412, Mach 2,
-    AUSM+-up + van Albada, CFL 0.5) run 2000 steps from the free stream with
-    exact and with fast math: the bow-shock solution agrees in Cl and Cd to
-    1e-4 and in the state to max-rel 1e-3 per conserved component (the
-    bound of the FP32 vs FP64 comparison above, which is how far any FP32
-    rounding choice moves this solution)."""
+def test_headline_2000_steps_bitwise_and_fast_within_the_rounding_floor():
+    """The single-grid headline configuration at full size (512 x 384 NACA
+    2412 in the reference's default rectangular far field, 10^4 Jacobi
+    sweeps, Mach 2, AUSM+-up + van Albada, CFL 0.5) run 2000 steps from the
+    free stream, into the developed bow shock.
+
+    Exact math (the headline arithmetic): the state after 2000 steps and the
+    residual of that state are bit-for-bit ref32, the restated WGSL -- the
+    per-step residual bound (max-rel 1e-5) holds with no error at all.
+
+    Fast math: Cl and Cd of its own 2000-step run within 1e-4 of exact
+    math's and the state within max-rel 1e-3; its residual of the developed
+    state differs from ref32's by no more than twice ref32's own rounding
+    error there (ref32 against the FP64 residual of the same state on the
+    same vertices, the FP64 solve being bit-for-bit ref64).  That floor is
+    above 1e-5 at this state, which is why fast math is not held to the
+    1e-5 field-wise bound."""
+    from oracle_lib import Oracle
+
     M, N, steps = 512, 384, 2000
     s = ausm.Solver(M, N)
     s.update_grid_boundaries(ausm.naca4_boundary(M, code=2412, size=0.3, x_offset=-0.7, exponent=1.4),
                              ausm.rect_boundary(M, 1.5))
-    s.prepare_grid(iters=2000)
+    s.prepare_grid(iters=10000)
     s.set_flow(2.0, 0.0)
     s.set_scheme(AUSMUP, VANALBADA, 0.0, 0.75)
     s.set_cfl(0.5, 1.0, 1)
-    out = {}
-    for mode in ("exact", "fast"):
-        s.set_math(mode)
-        s.prepare_state(freestream=True)
-        info = s.step(steps)
-        assert not info["nonfinite"]
-        out[mode] = (s.state().astype(np.float64)[1:N + 1], s.forces(ref_length=1.0))
-    (qe, fe), (qf, ff) = out["exact"], out["fast"]
+    s.set_tile(2)
+    vtx = s.vertices()
+    par = make_params(2.0, 0.0, cfl=0.5, maxdt=1.0)
+    s.set_math("exact")
+    s.prepare_state(freestream=True)
+    info = s.step(steps)
+    q = s.state()
+    fe = s.forces(ref_length=1.0)
+    ref = Solver(32, par, AUSMUP, VANALBADA, vtx, omp=True)
+    ref.step(steps)
+    assert np.array_equal(q[1:N + 1], ref.state[1:N + 1])
+    assert info["dt"] == ref.last_dt
+    r_exact = s.residual()
+    r_ref = Oracle(32, omp=True).residual(par, AUSMUP, VANALBADA, vtx, q)
+    assert np.array_equal(r_exact, r_ref)
+    # fast-math residual of the same state, and the FP64 residual
+    s.set_math("fast")
+    r_fast = s.residual()
+    s64 = ausm.Solver(M, N)
+    s64.set_vertices(vtx)
+    s64.set_flow(2.0, 0.0)
+    s64.set_scheme(AUSMUP, VANALBADA, 0.0, 0.75)
+    s64.set_precision("fp64")
+    s64.set_state(q)
+    r64 = s64.residual()
+    s64.close()
+
+    def max_rel(a, b):
+        a, b = a.astype(np.float64).reshape(-1, 4), b.astype(np.float64).reshape(-1, 4)
+        return float((np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)).max())
+
+    fast_vs_ref, ref_vs_64 = max_rel(r_fast, r_ref), max_rel(r_ref, r64)
+    # fast math's own 2000 steps
+    s.prepare_state(freestream=True)
+    s.step(steps)
+    qf = s.state().astype(np.float64)[1:N + 1]
+    ff = s.forces(ref_length=1.0)
+    qe = q.astype(np.float64)[1:N + 1]
     rel = (np.abs(qf - qe).max(axis=(0, 1)) / np.abs(qe).max(axis=(0, 1))).max()
-    print(f"headline 2000 steps: Cl {fe['cl']:.6f} / {ff['cl']:.6f}, Cd {fe['cd']:.6f} / {ff['cd']:.6f}, "
-          f"state max-rel {rel:.2e}")
+    print(f"headline 2000 steps: Cl {fe['cl']:.6f} / {ff['cl']:.6f}, Cd {fe['cd']:.6f} / "
+          f"{ff['cd']:.6f}, state max-rel {rel:.2e}; residual fast vs ref32 {fast_vs_ref:.2e}, "
+          f"ref32 vs FP64 {ref_vs_64:.2e}")
     assert abs(ff["cl"] - fe["cl"]) < 1e-4
     assert abs(ff["cd"] - fe["cd"]) < 1e-4
     assert rel < 1e-3
+    assert ref_vs_64 > 1e-6  # the floor is real, not zero
+    assert fast_vs_ref < 2 * ref_vs_64
     s.close()
