@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libausm.so")
 
 # enums (include/ausm.h)
 SLAU2, SLAU, AUSMUP = 0, 1, 2
-VANALBADA, MINMOD, VANLEER, NOLIMITER = 0, 1, 2, 3
+VANALBADA, MINMOD, VANLEER, NOLIMITER, VANALBADA_CLAMPED = 0, 1, 2, 3, 4
 MATH_EXACT, MATH_FAST = 0, 1
 PREC_FP32, PREC_FP64, PREC_FP32C = 0, 1, 2
 DT_GLOBAL, DT_LOCAL = 0, 1
@@ -25,7 +25,8 @@ GRID_JACOBI, GRID_SOR = 0, 1
 SPACING_POWER, SPACING_COSINE = 0, 1
 
 SCHEMES = {"SLAU2": SLAU2, "SLAU": SLAU, "AUSM+-up": AUSMUP}
-LIMITERS = {"vanAlbada": VANALBADA, "minmod": MINMOD, "vanLeer": VANLEER, "none": NOLIMITER}
+LIMITERS = {"vanAlbada": VANALBADA, "minmod": MINMOD, "vanLeer": VANLEER, "none": NOLIMITER,
+            "vanAlbadaClamped": VANALBADA_CLAMPED}
 
 ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "ORDER", -5: "NONFINITE", -6: "FULL",
           -7: "NOTFOUND"}

@@ -178,7 +178,7 @@ def solve(vtx, mach, aoa, prec="fp32", math_mode="exact", cfl=0.5, drop=1e-6,
 
 
 LIMITER_NAMES = {"vanalbada": L.VANALBADA, "minmod": L.MINMOD, "vanleer": L.VANLEER,
-                 "none": L.NOLIMITER}
+                 "none": L.NOLIMITER, "vanalbada_clamped": L.VANALBADA_CLAMPED}
 DEFAULT_RUNS = tuple((p, m, lim) for lim in ("vanalbada", "none")
                      for p, m in (("fp64", "exact"), ("fp32", "exact"), ("fp32", "fast"),
                                   ("fp32c", "exact"), ("fp32c", "fast")))

@@ -547,7 +547,7 @@ int ausm_set_freestream(ausm_ctx* c, double rho, double p, double vx, double vy,
 int ausm_set_scheme(ausm_ctx* c, int scheme, int limiter, float K_p, float K_u) {
   if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
   if (scheme < 0 || scheme > 2) return set_error(AUSM_ERR_ARG, "unknown scheme %d", scheme);
-  if (limiter < 0 || limiter > 3) return set_error(AUSM_ERR_ARG, "unknown limiter %d", limiter);
+  if (limiter < 0 || limiter > 4) return set_error(AUSM_ERR_ARG, "unknown limiter %d", limiter);
   hipSetDevice(c->device);
   if (scheme != c->scheme || limiter != c->limiter) drop_graphs(c);
   c->scheme = scheme;

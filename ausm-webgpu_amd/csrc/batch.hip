@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -166,6 +167,9 @@ struct ausm_batch {
   double* h_tmpd = nullptr;  // pinned
   float* d_stage = nullptr;  // get_state staging
   size_t stage_cap = 0;
+  double* d_seg = nullptr;   // per-segment forces of a retiring solve (2 M doubles)
+  size_t seg_cap = 0;
+  bool use_graphs = true;    // AUSM_NO_GRAPHS set: chunks are launched directly
   int max_items = 0;
   int n_items = 0;
   bool dirty = false;
@@ -181,6 +185,8 @@ struct ausm_batch {
 namespace ausm {
 
 static void drop_bgraphs(ausm_batch* b) {
+  // a launched graph may still be running on the stream
+  if (!b->graphs.empty()) (void)hipStreamSynchronize(b->stream);
   for (auto& g : b->graphs) {
     (void)hipGraphExecDestroy(g.e);
     (void)hipGraphDestroy(g.g);
@@ -458,6 +464,10 @@ static hipError_t enqueue_steps(ausm_batch* b, int nsteps) {
 
 static int run_chunk(ausm_batch* b, int nsteps) {
   const int blocks = b->cus * b->bpc;
+  if (!b->use_graphs) {
+    if (enqueue_steps(b, nsteps) != hipSuccess) return set_error(AUSM_ERR_HIP, "stage launch failed");
+    return AUSM_OK;
+  }
   BGraph* g = nullptr;
   for (auto& x : b->graphs)
     if (x.steps == nsteps && x.ty == b->ty && x.blocks == blocks && x.math == b->math &&
@@ -469,9 +479,14 @@ static int run_chunk(ausm_batch* b, int nsteps) {
       return set_error(AUSM_ERR_HIP, "graph capture failed");
     const hipError_t e = enqueue_steps(b, nsteps);
     const hipError_t e2 = hipStreamEndCapture(b->stream, &n.g);
-    if (e != hipSuccess || e2 != hipSuccess) return set_error(AUSM_ERR_HIP, "graph capture failed");
-    if (hipGraphInstantiate(&n.e, n.g, nullptr, nullptr, 0) != hipSuccess)
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (n.g) (void)hipGraphDestroy(n.g);
+      return set_error(AUSM_ERR_HIP, "graph capture failed");
+    }
+    if (hipGraphInstantiate(&n.e, n.g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(n.g);
       return set_error(AUSM_ERR_HIP, "graph instantiate failed");
+    }
     b->graphs.push_back(n);
     g = &b->graphs.back();
   }
@@ -483,16 +498,24 @@ static int retire(ausm_batch* b, Slot* s, int state) {
   s->state = state;
   // FP64 forces at completion (graphs.js:16-101)
   if (state != 4) {
-    double* seg = nullptr;
-    if (hipMalloc(&seg, sizeof(double) * 2 * s->d.M) != hipSuccess)
-      return set_error(AUSM_ERR_NOMEM, "force scratch allocation failed");
+    // grow-only segment buffer (a retiring solve does not allocate unless it
+    // is the largest seen so far)
+    const size_t need = 2 * (size_t)s->d.M;
+    if (need > b->seg_cap) {
+      (void)hipStreamSynchronize(b->stream);
+      (void)hipFree(b->d_seg);
+      b->d_seg = nullptr;
+      b->seg_cap = 0;
+      if (hipMalloc(&b->d_seg, sizeof(double) * need) != hipSuccess)
+        return set_error(AUSM_ERR_NOMEM, "force scratch allocation failed");
+      b->seg_cap = need;
+    }
     const double ref = s->d.ref_length > 0 ? s->d.ref_length : 1.0;
     hipError_t e = launch_forces(s->dev.q[0], s->dev.vx, s->dev.vy, s->dpar, s->d.M, s->dev.PS, ref,
-                                 seg, b->d_tmpd, nullptr, b->stream);
+                                 b->d_seg, b->d_tmpd, nullptr, b->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(b->h_tmpd, b->d_tmpd, 4 * sizeof(double), hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-    (void)hipFree(seg);
     if (e != hipSuccess) return set_error(AUSM_ERR_HIP, "forces failed");
     s->cl = b->h_tmpd[2];
     s->cd = b->h_tmpd[3];
@@ -554,6 +577,7 @@ int ausm_batch_create(int device, int max_solves, size_t budget, ausm_batch** ou
     return set_error(AUSM_ERR_NOMEM, "batch allocation failed (budget %zu bytes)", budget);
   }
   b->arena.init(b->arena_mem, budget);
+  b->use_graphs = getenv("AUSM_NO_GRAPHS") == nullptr;
   b->table.assign(max_solves, nullptr);
   b->h_solves.assign(max_solves, SolveDev{});
   *out = b;
@@ -574,6 +598,7 @@ int ausm_batch_destroy(ausm_batch* b) {
   (void)hipFree(b->d_tmp);
   (void)hipFree(b->d_tmpd);
   (void)hipFree(b->d_stage);
+  (void)hipFree(b->d_seg);
   if (b->h_out) (void)hipHostFree(b->h_out);
   if (b->h_tmpd) (void)hipHostFree(b->h_tmpd);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -625,7 +650,7 @@ int ausm_batch_submit(ausm_batch* b, const ausm_solve_desc* d, uint64_t* id) {
   if (!b || !d || !id) return set_error(AUSM_ERR_ARG, "NULL argument");
   if (d->M < 4 || d->N < 4 || d->M > (1 << 15) || d->N > (1 << 15) || !d->vertices)
     return set_error(AUSM_ERR_ARG, "bad grid in solve descriptor");
-  if (d->scheme < 0 || d->scheme > 2 || d->limiter < 0 || d->limiter > 3)
+  if (d->scheme < 0 || d->scheme > 2 || d->limiter < 0 || d->limiter > 4)
     return set_error(AUSM_ERR_ARG, "bad scheme or limiter");
   if (!(d->rho > 0) || !(d->p > 0) || !(d->gamma > 1) || !(d->cfl > 0) || d->max_steps < 1)
     return set_error(AUSM_ERR_ARG, "bad flow or run settings");

@@ -21,7 +21,7 @@
 namespace ausm {
 
 enum { SCHEME_SLAU2 = 0, SCHEME_SLAU = 1, SCHEME_AUSMUP = 2 };
-enum { LIM_VANALBADA = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_NONE = 3 };
+enum { LIM_VANALBADA = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_NONE = 3, LIM_VANALBADA_CLAMPED = 4 };
 
 // Solver parameters resident in device memory so that captured graphs pick
 // up changes without being rebuilt (the reference's Uniforms buffer,
@@ -143,8 +143,13 @@ __device__ __forceinline__ void muscl_exact(int limiter, R wl, R wr, R dL, R dF,
   } else {
     const R rL = (dL * dF + eps) / (dF * dF + eps);
     const R rR = (dF * dR + eps) / (dF * dF + eps);
-    const R phiL = limiter == LIM_VANLEER ? van_leer_exact(rL) : van_albada_exact(rL);
-    const R phiR = limiter == LIM_VANLEER ? van_leer_exact(rR) : van_albada_exact(rR);
+    R phiL = limiter == LIM_VANLEER ? van_leer_exact(rL) : van_albada_exact(rL);
+    R phiR = limiter == LIM_VANLEER ? van_leer_exact(rR) : van_albada_exact(rR);
+    if (limiter == LIM_VANALBADA_CLAMPED) {
+      // van Albada as a TVD limiter: no reconstruction where r <= 0
+      phiL = rL > R(0) ? phiL : R(0);
+      phiR = rR > R(0) ? phiR : R(0);
+    }
     L = wl + (R(0.5) * phiL) * dF;
     Rr = wr - (R(0.5) * phiR) * dF;
   }
@@ -288,8 +293,14 @@ __device__ __forceinline__ void muscl_fast(int limiter, float wl, float wr, floa
     const float nL = fmaf(dL, dF, eps), nR = fmaf(dF, dR, eps);
     const float denL = fmaf(nL, nL, d2), denR = fmaf(nR, nR, d2);
     const float rh = __builtin_amdgcn_rcpf(denL * denR) * hF;
-    L = fmaf(nL * (nL + d), denR * rh, wl);
-    Rr = fmaf(-(nR * (nR + d)), denL * rh, wr);
+    float aL = nL * (nL + d), aR = nR * (nR + d);
+    if (limiter == LIM_VANALBADA_CLAMPED) {
+      // r > 0 exactly when n > 0 (d > 0): the clamped limiter's phi = 0
+      aL = nL > 0.0f ? aL : 0.0f;
+      aR = nR > 0.0f ? aR : 0.0f;
+    }
+    L = fmaf(aL, denR * rh, wl);
+    Rr = fmaf(-aR, denL * rh, wr);
   }
 }
 

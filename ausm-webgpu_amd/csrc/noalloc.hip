@@ -119,7 +119,7 @@ extern "C" int ausm_step_noalloc(int device, const ausm_step_desc* d, const floa
   if (M < 4 || N < 4 || M > (1 << 15) || N > (1 << 15))
     return set_error(AUSM_ERR_ARG, "grid size %d x %d out of range (4..32768)", M, N);
   if (nsteps < 0) return set_error(AUSM_ERR_ARG, "negative step count");
-  if (d->scheme < 0 || d->scheme > 2 || d->limiter < 0 || d->limiter > 3)
+  if (d->scheme < 0 || d->scheme > 2 || d->limiter < 0 || d->limiter > 4)
     return set_error(AUSM_ERR_ARG, "unknown scheme %d or limiter %d", d->scheme, d->limiter);
   if (d->math != AUSM_MATH_EXACT && d->math != AUSM_MATH_FAST)
     return set_error(AUSM_ERR_ARG, "unknown math mode %d", d->math);

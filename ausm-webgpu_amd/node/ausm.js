@@ -11,7 +11,7 @@ const native = require(path.join(__dirname, 'ausm_napi.node'));
 
 // flux scheme table, in the reference order (src/util.js:114-118)
 const solvers = Object.freeze({ 'SLAU2': 0, 'SLAU': 1, 'AUSM+-up': 2 });
-const limiters = Object.freeze({ vanAlbada: 0, minmod: 1, vanLeer: 2, none: 3 });
+const limiters = Object.freeze({ vanAlbada: 0, minmod: 1, vanLeer: 2, none: 3, vanAlbadaClamped: 4 });
 const INIT_REST = 0;
 const INIT_FREESTREAM = 1;
 

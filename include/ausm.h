@@ -50,8 +50,14 @@ enum ausm_scheme { AUSM_SLAU2 = 0, AUSM_SLAU = 1, AUSM_AUSMUP = 2 };
 /* MUSCL limiters.  VANALBADA is the one the reference wires
  * (src/shaders/ausm.js:247-248); MINMOD and VANLEER are the reference's
  * unwired limiter functions (ausm.js:159-166); NONE is uni.muscl = 0
- * (first order, ausm.js:240). */
-enum ausm_limiter { AUSM_VANALBADA = 0, AUSM_MINMOD = 1, AUSM_VANLEER = 2, AUSM_NOLIMITER = 3 };
+ * (first order, ausm.js:240).  VANALBADA_CLAMPED has no counterpart in the
+ * reference: van Albada as a TVD limiter, phi(r) = (r^2+r)/(r^2+1) for
+ * r > 0 and 0 for r <= 0 (the reference's form goes negative for
+ * -1 < r < 0), a convergence option for steady solves. */
+enum ausm_limiter {
+  AUSM_VANALBADA = 0, AUSM_MINMOD = 1, AUSM_VANLEER = 2, AUSM_NOLIMITER = 3,
+  AUSM_VANALBADA_CLAMPED = 4
+};
 
 /* arithmetic of the stage kernels: EXACT follows the reference WGSL
  * expression by expression with IEEE division/sqrt and no contraction;

@@ -61,7 +61,7 @@ enum {
   P_MAXDT = 9, P_CFL = 10, P_KP = 11, P_KU = 12, P_COUNT = 13
 };
 enum { SCHEME_SLAU2 = 0, SCHEME_SLAU = 1, SCHEME_AUSMUP = 2 }; /* util.js:114-118 */
-enum { LIM_VANALBADA = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_NONE = 3 };
+enum { LIM_VANALBADA = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_NONE = 3, LIM_VANALBADA_CLAMPED = 4 };
 
 static inline R rmin(R a, R b) { return sizeof(R) == 4 ? (R)fminf((float)a, (float)b) : (R)fmin(a, b); }
 static inline R rmax(R a, R b) { return sizeof(R) == 4 ? (R)fmaxf((float)a, (float)b) : (R)fmax(a, b); }
@@ -140,6 +140,12 @@ void FN(face_flux)(const R* par, int scheme, int limiter, const R* QL2, const R*
         R rR = (dF * dR + eps) / (dF * dF + eps);
         R phiL = limiter == LIM_VANLEER ? van_leer(rL) : van_albada(rL);
         R phiR = limiter == LIM_VANLEER ? van_leer(rR) : van_albada(rR);
+        if (limiter == LIM_VANALBADA_CLAMPED) {
+          /* build extension (no counterpart in the reference): van Albada
+           * as a TVD limiter, phi = 0 for r <= 0 */
+          if (!(rL > (R)0)) phiL = (R)0;
+          if (!(rR > (R)0)) phiR = (R)0;
+        }
         L[c] = L[c] + ((R)0.5 * phiL) * dF;
         Rr[c] = Rr[c] - ((R)0.5 * phiR) * dF;
       }

@@ -16,7 +16,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIBORC = os.path.join(ORACLE_DIR, "liborc.so")
 
 SLAU2, SLAU, AUSMUP = 0, 1, 2
-VANALBADA, MINMOD, VANLEER, NONE = 0, 1, 2, 3
+VANALBADA, MINMOD, VANLEER, NONE, VANALBADA_CLAMPED = 0, 1, 2, 3, 4
 SCHEMES = {"SLAU2": SLAU2, "SLAU": SLAU, "AUSM+-up": AUSMUP}
 
 P_GAMMA, P_INRHO, P_INP, P_VX, P_VY = 0, 1, 2, 3, 4

@@ -14,13 +14,14 @@ import numpy as np
 import pytest
 
 import ausm
-from oracle_lib import (AUSMUP, MINMOD, NONE, SLAU, SLAU2, VANALBADA, VANLEER, Oracle, Solver,
+from oracle_lib import (AUSMUP, MINMOD, NONE, SLAU, SLAU2, VANALBADA, VANALBADA_CLAMPED, VANLEER,
+                        Oracle, Solver,
                         circle_boundary, make_params64, naca4_boundary, naca4_digits)
 
 pytestmark = pytest.mark.gpu
 
 SCHEMES = [SLAU2, SLAU, AUSMUP]
-LIMITERS = [VANALBADA, MINMOD, VANLEER, NONE]
+LIMITERS = [VANALBADA, MINMOD, VANLEER, NONE, VANALBADA_CLAMPED]
 
 
 def grid64(M, N, code=2412, iters=150, radius=10.0):

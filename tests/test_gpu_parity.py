@@ -20,13 +20,14 @@ import numpy as np
 import pytest
 
 import ausm
-from oracle_lib import (AUSMUP, MINMOD, NONE, SLAU, SLAU2, VANALBADA, VANLEER, Oracle, Solver,
+from oracle_lib import (AUSMUP, MINMOD, NONE, SLAU, SLAU2, VANALBADA, VANALBADA_CLAMPED, VANLEER,
+                        Oracle, Solver,
                         circle_boundary, make_params, naca4_boundary, naca4_digits, rect_boundary)
 
 pytestmark = pytest.mark.gpu
 
 SCHEMES = [SLAU2, SLAU, AUSMUP]
-LIMITERS = [VANALBADA, MINMOD, VANLEER, NONE]
+LIMITERS = [VANALBADA, MINMOD, VANLEER, NONE, VANALBADA_CLAMPED]
 FAST_TOL_NORM = 1e-5
 FAST_TOL_FIELD = 1e-4
 

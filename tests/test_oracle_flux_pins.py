@@ -37,6 +37,9 @@ fails it: a K_u term of the wrong sign or scale, a K_p term of the wrong
 sign, rho_inf for rho_L in M_inf^2, the SLAU2 Ubar term scaled, the SLAU
 (1 - chi) term scaled, the sign of the chi dp term, the (1 - g) factor
 dropped, and a wrong isentropic exponent in the ghost density.
+
+The last pin covers a build extension, the clamped (TVD) van Albada
+limiter, which has no counterpart in the reference.
 """
 import ctypes
 import math
@@ -504,3 +507,44 @@ def test_sod_exact_solution_textbook_values():
     rho, p, u = sod_exact(x, 0.2)
     assert abs(p - 0.30313) < 5e-5 and abs(u - 0.92745) < 5e-5
     assert abs(rho[0] - 0.42632) < 5e-5 and abs(rho[1] - 0.26557) < 5e-5
+
+
+# ----------------------------------------------------------------------------
+# clamped van Albada (a build extension: the TVD form of the limiter)
+# ----------------------------------------------------------------------------
+
+def test_clamped_van_albada_is_tvd_van_albada():
+    """VANALBADA_CLAMPED is phi(r) = (r^2+r)/(r^2+1) for r > 0 and 0 for
+    r <= 0.  On monotone stencils (r > 0 on both sides) it is the
+    reference's van Albada bit for bit; at an extremum (r < 0 on both sides,
+    both inside -1 < r < 0 where the reference's phi is negative and beyond
+    r < -1 where it is positive again) there is no reconstruction: the flux
+    is the first-order flux.  With r < 0 on one side only, that side alone
+    falls back to first order."""
+    from oracle_lib import VANALBADA_CLAMPED
+
+    o = Oracle(64)
+    par = params(0.5)
+    n = (1.0, 0.0)
+    p = 0.8
+    rhos_mono = (1.0, 1.1, 1.25, 1.3)              # r_L = 0.67, r_R = 0.33
+    rhos_ext_a = (1.0, 1.2, 1.1, 1.2)              # r_L = -2, r_R = -1
+    rhos_ext_b = (1.2, 1.0, 1.4, 1.2)              # r_L = -0.5, r_R = -0.5
+    for scheme in (SLAU2, SLAU, AUSMUP):
+        Q = [cons(r, 0.3, 0.1, p) for r in rhos_mono]
+        assert np.array_equal(o.face_flux(par, scheme, VANALBADA_CLAMPED, *Q, n),
+                              o.face_flux(par, scheme, VANALBADA, *Q, n))
+        for rhos in (rhos_ext_a, rhos_ext_b):
+            Q = [cons(r, 0.3, 0.1, p) for r in rhos]
+            Fc = o.face_flux(par, scheme, VANALBADA_CLAMPED, *Q, n)
+            F1 = o.face_flux(par, scheme, NONE, *Q, n)
+            Fv = o.face_flux(par, scheme, VANALBADA, *Q, n)
+            assert np.array_equal(Fc, F1), rhos
+            assert not np.allclose(Fv, F1, rtol=1e-6), rhos  # the reference's phi != 0 here
+    # one-sided: r_L = 0.5 > 0 (reconstructed), r_R = -0.5 (first order);
+    # the flow runs from right to left, so the right state is convected and
+    # the flux differs from both the first-order and the reference's flux
+    Q = [cons(r, -0.3, 0.0, p) for r in (1.0, 1.1, 1.3, 1.2)]
+    Fc = o.face_flux(par, SLAU2, VANALBADA_CLAMPED, *Q, n)
+    assert not np.array_equal(Fc, o.face_flux(par, SLAU2, NONE, *Q, n))
+    assert not np.array_equal(Fc, o.face_flux(par, SLAU2, VANALBADA, *Q, n))
