@@ -241,7 +241,9 @@ def residual_parity(ausm, s, q):
                  "residual is exact_vs_fp64 away from the FP64 residual)"}
 
 
-def probe(ausm, ev, M, N, args):
+def calibration_copies(ev):
+    """Three 512 MiB device-to-device copies: the kernel of known traffic
+    the FETCH_SIZE / WRITE_SIZE counters are scaled against."""
     H = ev.L
     H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
@@ -256,6 +258,11 @@ def probe(ausm, ev, M, N, args):
     ev.device_sync()
     H.hipFree(a)
     H.hipFree(b)
+    return n
+
+
+def probe(ausm, ev, M, N, args):
+    n = calibration_copies(ev)
     s, _ = build_workload(ausm, M, N, args.grid_iters)
     s.set_math(args.math)
     s.prepare_state(freestream=True)
@@ -265,8 +272,29 @@ def probe(ausm, ev, M, N, args):
     s.close()
 
 
+def probe_batched(ausm, ev, args):
+    """Counter-collection run of the batched engine: the calibration copies,
+    then the C3 mix (grids from --grid-iters sweeps) advanced --batch-steps
+    steps in fast math, no timing and no JSON."""
+    from ausm import serve
+
+    n = calibration_copies(ev)
+    reqs = serve.make_requests(args.batch_solves, seed=2026, sizes=BATCH_SIZES,
+                               max_steps=args.batch_steps, res_drop=0.0)
+    serve.make_grids(reqs, iters=args.grid_iters)
+    with ausm.Batch(max_solves=len(reqs), math=args.math, precision="fp32") as b:
+        for r in reqs:
+            b.submit(r["vertices"], ausm.freestream(r["mach"], r["aoa"]), scheme=r["scheme"],
+                     cfl=r["cfl"], max_steps=args.batch_steps, dt_mode="global")
+        b.advance(args.batch_steps, args.batch_steps)
+    log(f"[bench] batched probe: 3 x {n} B copies, {len(reqs)} solves x {args.batch_steps} steps")
+
+
+BATCH_SIZES = [(256, 128), (384, 192), (512, 384), (768, 512)]
+
+
 def batched_leg(serve, n, steps, grid_iters):
-    sizes = [(256, 128), (384, 192), (512, 384), (768, 512)]
+    sizes = BATCH_SIZES
     reqs = serve.make_requests(n, seed=2026, sizes=sizes, max_steps=steps, res_drop=0.0)
     serve.make_grids(reqs, iters=grid_iters)
     log(f"[bench] batched: {n} solves x {steps} steps")
@@ -281,12 +309,21 @@ def batched_leg(serve, n, steps, grid_iters):
                 bat["results"][r["id"]]["res_l2"] == seq["results"][r["id"]]["res_l2"])
                for r in reqs)
     strip = lambda d: {k: v for k, v in d.items() if k != "results"}
+    # the whole batched run on the 152-B accounting (persistent stage
+    # kernels plus the per-step dt kernel, convergence checks and admission)
+    ach = ALG_BYTES_PER_CELL_STEP * bat["cell_steps_per_s"] / 1e9
     return {"workload": f"{n} solves, sizes from {sizes}, NACA m~U[0,.06] p~U[.2,.6] "
                         f"t~U[.08,.18], Mach~U[0.3,3], AoA~U[-4,8], scheme~U{{SLAU2,SLAU,AUSM+-up}}, "
                         f"{steps} steps each",
             "batched": strip(bat), "one_by_one": strip(seq),
             "speedup_solves_per_s": bat["solves_per_s"] / seq["solves_per_s"],
-            "results_identical": same}
+            "results_identical": same,
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBPS,
+                         "kernel": "ausm::batch_stage_kernel (persistent fused RK stage, fast math)",
+                         "source": "152 B per cell-step x the batched run's cell-steps/s (whole "
+                                   "run, makespan timed on the host); per-launch durations in "
+                                   "profiles/batch_stage_kernel_c3.md"}}
 
 
 SERVE_VARIANTS = (("MUSCL van Albada (the reference's reconstruction)", 0),
@@ -399,6 +436,10 @@ def main():
                     help="counter-collection run: three 512 MiB device copies (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE) then --steps steps of the workload, "
                          "no timing and no JSON")
+    ap.add_argument("--probe-batched", action="store_true",
+                    help="counter-collection run of the batched engine: the calibration copies, "
+                         "then the batched mix (grids from --grid-iters sweeps) advanced "
+                         "--batch-steps steps, no timing and no JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -413,6 +454,9 @@ def main():
     ev = HipEvents()
     if args.probe:
         probe(ausm, ev, M, N, args)
+        return
+    if args.probe_batched:
+        probe_batched(ausm, ev, args)
         return
     log(f"[bench] building {M}x{N} NACA 2412 grid ({args.grid_iters} FP64 Jacobi sweeps)")
     s, t_grid = build_workload(ausm, M, N, args.grid_iters)
