@@ -64,10 +64,12 @@ class Batch:
 
     def submit(self, vertices, freestream, scheme=L.SLAU2, limiter=L.VANALBADA, K_p=0.0,
                K_u=0.75, cfl=0.5, maxdt=1.0, init="freestream", max_steps=1000, res_drop=0.0,
-               ref_length=1.0, dt_mode="global"):
+               ref_length=1.0, dt_mode="global", freeze_after=0):
         """Queue one solve.  vertices: (N+1, M, 2) FP32; freestream: (rho, p,
         vx, vy, gamma); dt_mode "global" (the reference's) or "local" (each
-        cell its own CFL step).  Returns the solve id."""
+        cell its own CFL step); freeze_after > 0 freezes a van Albada limiter
+        (Solver.freeze_limiter) at the first convergence check at or after
+        that many steps.  Returns the solve id."""
         v = f32(vertices)
         N1, M, _ = v.shape
         rho, p, vx, vy, gamma = freestream
@@ -77,7 +79,8 @@ class Batch:
                         scheme=scheme, limiter=limiter, K_p=K_p, K_u=K_u, cfl=cfl, maxdt=maxdt,
                         init_mode=L.INIT_FREESTREAM if init == "freestream" else L.INIT_REST,
                         max_steps=max_steps, res_drop=res_drop, ref_length=ref_length,
-                        dt_mode=L.DT_LOCAL if dt_mode == "local" else L.DT_GLOBAL)
+                        dt_mode=L.DT_LOCAL if dt_mode == "local" else L.DT_GLOBAL,
+                        freeze_after=freeze_after)
         sid = ctypes.c_uint64()
         check(self._L.ausm_batch_submit(self.h, ctypes.byref(d), ctypes.byref(sid)))
         self._keep[sid.value] = (M, N1 - 1)

@@ -54,7 +54,7 @@ class SolveDesc(ctypes.Structure):
                 ("cfl", ctypes.c_float), ("maxdt", ctypes.c_float),
                 ("init_mode", ctypes.c_int), ("max_steps", ctypes.c_int),
                 ("res_drop", ctypes.c_double), ("ref_length", ctypes.c_double),
-                ("dt_mode", ctypes.c_int)]
+                ("dt_mode", ctypes.c_int), ("freeze_after", ctypes.c_int)]
 
 
 class StepDesc(ctypes.Structure):
@@ -106,6 +106,8 @@ SIGNATURES = {
     "ausm_set_tile": (_i, [_vp, _i]),
     "ausm_set_precision": (_i, [_vp, _i]),
     "ausm_set_dt_mode": (_i, [_vp, _i]),
+    "ausm_freeze_limiter": (_i, [_vp, _i]),
+    "ausm_limiter_frozen": (_i, [_vp, _vp]),
     "ausm_init_state": (_i, [_vp, _i]),
     "ausm_set_state": (_i, [_vp, _vp]),
     "ausm_get_state": (_i, [_vp, _vp]),

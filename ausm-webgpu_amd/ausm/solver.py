@@ -236,6 +236,21 @@ class Solver:
             mode = {"global": L.DT_GLOBAL, "local": L.DT_LOCAL}[mode]
         check(self._L.ausm_set_dt_mode(self.h, mode))
 
+    def freeze_limiter(self, on=True):
+        """Limiter freezing (van Albada limiters; no counterpart in the
+        reference): record psi = phi(r)/r of every face from the current
+        state and reconstruct with it from now on, until freeze_limiter(False)
+        or a new state, grid, limiter or precision.  A convergence device for
+        MUSCL solves whose limiter switching holds the residual in a limit
+        cycle."""
+        check(self._L.ausm_freeze_limiter(self.h, 1 if on else 0))
+
+    @property
+    def limiter_frozen(self):
+        on = ctypes.c_int()
+        check(self._L.ausm_limiter_frozen(self.h, ctypes.byref(on)))
+        return bool(on.value)
+
     def set_precision(self, prec):
         """"fp32" (the reference's arithmetic), "fp64" (the comparison solve)
         or "fp32c" (FP32 arithmetic, compensated hi + lo state accumulation);

@@ -118,6 +118,8 @@ StageArgs64 stage_args64(ausm_ctx* c, int ty, int stage) {
   a.qin = c->q64[0];
   a.qn = c->q64[0];
   a.qout = c->q64[1];
+  a.freeze = c->freeze;
+  a.psi = (double*)c->psi;
   return a;
 }
 
@@ -221,6 +223,8 @@ StageArgs stage_args(ausm_ctx* c, int ty, int stage) {
   a.qin = c->q[0];
   a.qn = c->q[0];
   a.qout = c->q[1];
+  a.freeze = c->freeze;
+  a.psi = (float*)c->psi;
   return a;
 }
 
@@ -257,6 +261,14 @@ void drop_graphs(ausm_ctx* c) {
     hipGraphDestroy(g.graph);
   }
   c->graphs.clear();
+}
+
+// a new state, grid, limiter or precision ends limiter freezing (the
+// recorded psi belongs to the state it was taken from)
+static void unfreeze(ausm_ctx* c) {
+  if (!c->freeze) return;
+  drop_graphs(c);  // captured stages carry the freeze mode
+  c->freeze = 0;
 }
 
 }  // namespace ausm
@@ -360,6 +372,7 @@ int ausm_ctx_destroy(ausm_ctx* c) {
   hipFree(c->stage_buf64);
   hipFree(c->acc);
   hipFree(c->lo);
+  hipFree(c->psi);
   hipFree(c->stage_buf);
   hipFree(c->seg);
   if (c->pinned) hipHostFree(c->pinned);
@@ -385,6 +398,7 @@ static int finish_grid(ausm_ctx* c, const double* gx, const double* gy) {
   // captured FP64 step graphs hold the previous grid's vertex pointers
   // (g64[g_cur], which moves between the two ping-pong pairs)
   drop_graphs(c);
+  unfreeze(c);
   const size_t nv = (size_t)(c->N + 1) * c->P;
   CHECK_HIP(launch_f64_to_f32(gx, c->vx, nv, c->stream), "grid convert");
   CHECK_HIP(launch_f64_to_f32(gy, c->vy, nv, c->stream), "grid convert");
@@ -452,6 +466,7 @@ int ausm_grid_set_vertices(ausm_ctx* c, const float* xy) {
   CHECK_HIP(hipMemcpyAsync(c->g64[1], dy.data(), sizeof(double) * dy.size(), hipMemcpyHostToDevice, c->stream), "vertex upload");
   CHECK_HIP(hipStreamSynchronize(c->stream), "vertex upload");
   drop_graphs(c);  // FP64 graphs may hold the other vertex pair
+  unfreeze(c);
   c->has_grid = true;
   c->has_state = false;
   c->g_cur = 0;
@@ -549,7 +564,10 @@ int ausm_set_scheme(ausm_ctx* c, int scheme, int limiter, float K_p, float K_u) 
   if (scheme < 0 || scheme > 2) return set_error(AUSM_ERR_ARG, "unknown scheme %d", scheme);
   if (limiter < 0 || limiter > 4) return set_error(AUSM_ERR_ARG, "unknown limiter %d", limiter);
   hipSetDevice(c->device);
-  if (scheme != c->scheme || limiter != c->limiter) drop_graphs(c);
+  if (scheme != c->scheme || limiter != c->limiter) {
+    drop_graphs(c);
+    unfreeze(c);
+  }
   c->scheme = scheme;
   c->limiter = limiter;
   c->hpar.Kp = K_p;
@@ -596,6 +614,44 @@ int ausm_set_tile(ausm_ctx* c, int rows) {
   return AUSM_OK;
 }
 
+// ---- limiter freezing ----------------------------------------------------------
+// Records psi = phi(r)/r of every face and component from the current state
+// (a residual pass of the general stage kernel) and makes every later stage
+// reconstruct with it, until on = 0 or a new state, grid, limiter or
+// precision.  A build extension with no counterpart in the reference.
+static int current_rows(ausm_ctx* c);
+
+int ausm_freeze_limiter(ausm_ctx* c, int on) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (on != 0 && on != 1) return set_error(AUSM_ERR_ARG, "freeze flag must be 0 or 1");
+  hipSetDevice(c->device);
+  if (!on) {
+    unfreeze(c);
+    return AUSM_OK;
+  }
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "no state to record the limiter from");
+  if (c->limiter != LIM_VANALBADA && c->limiter != LIM_VANALBADA_CLAMPED)
+    return set_error(AUSM_ERR_ARG, "limiter freezing needs a van Albada limiter (limiter %d)", c->limiter);
+  if (!c->psi && hipMalloc(&c->psi, sizeof(double) * 2 * kPsiPerFace * psi_plane(c->N, c->P)) != hipSuccess) {
+    c->psi = nullptr;
+    return set_error(AUSM_ERR_NOMEM, "device allocation failed for the limiter record");
+  }
+  const int ty = current_rows(c);  // tuned (if at all) before recording
+  drop_graphs(c);
+  c->freeze = 1;
+  const hipError_t e = stage_launch(c, ty, 0, 0, 0, 2, false, false, false);
+  c->freeze = e == hipSuccess ? 2 : 0;
+  CHECK_HIP(e, "limiter record");
+  CHECK_HIP(hipStreamSynchronize(c->stream), "limiter record");
+  return AUSM_OK;
+}
+
+int ausm_limiter_frozen(ausm_ctx* c, int* on) {
+  if (!c || !on) return set_error(AUSM_ERR_ARG, "NULL argument");
+  *on = c->freeze ? 1 : 0;
+  return AUSM_OK;
+}
+
 // ---- state --------------------------------------------------------------------
 static int refresh_ws(ausm_ctx* c) {
   // wave speeds of the current solution -> per-block maxima (the post pass
@@ -619,6 +675,7 @@ int ausm_init_state(ausm_ctx* c, int mode) {
   if (mode != AUSM_INIT_REST && mode != AUSM_INIT_FREESTREAM)
     return set_error(AUSM_ERR_ARG, "unknown init mode %d", mode);
   hipSetDevice(c->device);
+  unfreeze(c);
   if (fp64(c))
     CHECK_HIP(launch_fill_state(c->q64[0], c->dpar64, c->M, c->N, c->P, c->PS, mode, c->stream), "init state");
   else
@@ -637,6 +694,7 @@ int ausm_set_state(ausm_ctx* c, const float* q) {
   if (!c || !q) return set_error(AUSM_ERR_ARG, "NULL argument");
   if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "set a grid before the state");
   hipSetDevice(c->device);
+  unfreeze(c);
   const size_t n = 4 * (size_t)c->M * (c->N + 3);
   CHECK_HIP(hipMemcpyAsync(c->stage_buf, q, sizeof(float) * n, hipMemcpyHostToDevice, c->stream), "state upload");
   if (fp64(c)) {
@@ -720,6 +778,7 @@ int ausm_set_precision(ausm_ctx* c, int precision) {
   if (r == AUSM_OK && precision == AUSM_PREC_FP32C) r = ensure_comp_buffers(c);
   if (r != AUSM_OK) return r;
   drop_graphs(c);
+  unfreeze(c);
   // carry the current solution over to the new precision
   if (c->has_state) {
     const int M = c->M, N = c->N, P = c->P, PS = c->PS;
@@ -752,6 +811,7 @@ int ausm_set_state64(ausm_ctx* c, const double* q) {
     return set_error(AUSM_ERR_ORDER, "FP64 state needs FP64 or compensated precision");
   if (!c->has_grid) return set_error(AUSM_ERR_ORDER, "set a grid before the state");
   hipSetDevice(c->device);
+  unfreeze(c);
   const int rb = ensure_fp64_buffers(c);
   if (rb != AUSM_OK) return rb;
   const size_t n = 4 * (size_t)c->M * (c->N + 3);

@@ -105,6 +105,7 @@ struct Slot {
   double res = -1.0, res0 = -1.0, cl = 0.0, cd = 0.0;
   size_t off = 0, bytes = 0;
   int index = -1;
+  bool frozen = false;  // limiter frozen (freeze_after reached)
   SolveDev dev{};
   DevParams* dpar = nullptr;  // the solve's own parameter copy in the arena
 };
@@ -118,13 +119,14 @@ struct BGraph {
 // sizes of one solve's arena block
 struct Layout {
   int P, PS, tiles_x, nws_max;
-  size_t q, v, ws, res, par, comp, total;
+  size_t q, v, ws, res, par, comp, psi, total;
 };
 
 static const int kBatchTiles[] = {1, 2, 4};
 
-// comp: two more state-sized plane sets (accumulator, low part)
-static Layout layout_of(int M, int N, bool comp) {
+// comp: two more state-sized plane sets (accumulator, low part); frz: the
+// limiter-freezing psi planes
+static Layout layout_of(int M, int N, bool comp, bool frz = false) {
   Layout L{};
   L.P = (int)rup((size_t)M, 64);
   L.PS = (int)rup((size_t)N * L.P, 64);
@@ -138,7 +140,8 @@ static Layout layout_of(int M, int N, bool comp) {
   L.res = rup(sizeof(double) * (size_t)N * L.tiles_x, 256);
   L.par = rup(sizeof(DevParams) + sizeof(float), 256);
   L.comp = comp ? 2 * L.q : 0;
-  L.total = 3 * L.q + 2 * L.v + L.ws + L.res + L.par + L.comp;
+  L.psi = frz ? rup(sizeof(float) * 2 * kPsiPerFace * psi_plane(N, L.P), 256) : 0;
+  L.total = 3 * L.q + 2 * L.v + L.ws + L.res + L.par + L.comp + L.psi;
   return L;
 }
 
@@ -284,7 +287,7 @@ static int admit(ausm_batch* b) {
     if (idx < 0) return AUSM_OK;
     Slot* s = b->queue.front();
     const int M = s->d.M, N = s->d.N;
-    const Layout L = layout_of(M, N, b->comp);
+    const Layout L = layout_of(M, N, b->comp, s->d.freeze_after > 0);
     size_t off;
     if (!b->arena.alloc(L.total, off)) return AUSM_OK;  // wait for memory
     b->queue.pop_front();
@@ -308,6 +311,7 @@ static int admit(ausm_batch* b) {
       D.acc = (float*)pc;
       D.lo = (float*)(pc + L.q);
     }
+    if (L.psi) D.psi = (float*)(p + 3 * L.q + 2 * L.v + L.ws + L.res + L.par + L.comp);
     D.M = M;
     D.N = N;
     D.P = L.P;
@@ -527,6 +531,28 @@ static int retire(ausm_batch* b, Slot* s, int state) {
   return AUSM_OK;
 }
 
+// limiter freezing of a running solve: psi recorded from its current
+// state by a residual pass of the general stage kernel into the scratch Q2
+// (the same record as ausm_freeze_limiter), then the solve's table entry
+// switched to the frozen reconstruction
+static int freeze_solve(ausm_batch* b, Slot* s) {
+  StageArgs a = batch_args(s->dev, 4, 0, 0);  // Q^n in, scratch Q2 out
+  a.par = s->dpar;
+  a.stage = 0;
+  a.freeze = 1;
+  a.local_dt = 0;
+  if (launch_stage(a, s->d.scheme, b->math, 2, b->stream) != hipSuccess)
+    return set_error(AUSM_ERR_HIP, "limiter record failed");
+  s->frozen = true;
+  s->dev.freeze = 2;
+  b->h_solves[s->index].freeze = 2;
+  if (hipMemcpyAsync(b->d_solves + s->index, &b->h_solves[s->index], sizeof(SolveDev),
+                     hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return set_error(AUSM_ERR_HIP, "limiter record failed");
+  return AUSM_OK;
+}
+
 static void free_slot(ausm_batch* b, Slot* s) {
   if (s->index >= 0) {
     b->h_solves[s->index].active = 0;
@@ -658,7 +684,10 @@ int ausm_batch_submit(ausm_batch* b, const ausm_solve_desc* d, uint64_t* id) {
     return set_error(AUSM_ERR_ARG, "bad init mode");
   if (d->dt_mode != AUSM_DT_GLOBAL && d->dt_mode != AUSM_DT_LOCAL)
     return set_error(AUSM_ERR_ARG, "bad dt mode");
-  const Layout L = layout_of(d->M, d->N, b->comp);
+  if (d->freeze_after < 0 ||
+      (d->freeze_after > 0 && d->limiter != LIM_VANALBADA && d->limiter != LIM_VANALBADA_CLAMPED))
+    return set_error(AUSM_ERR_ARG, "limiter freezing needs a van Albada limiter");
+  const Layout L = layout_of(d->M, d->N, b->comp, d->freeze_after > 0);
   if (L.total > b->arena.size())
     return set_error(AUSM_ERR_FULL, "a %d x %d solve needs %zu bytes, more than the whole budget", d->M, d->N, L.total);
   Slot* s = new Slot();
@@ -704,6 +733,9 @@ int ausm_batch_advance(ausm_batch* b, int nsteps, int check_every, int* pending)
       else if (s->steps >= s->d.max_steps) st = 3;
       if (st) {
         r = retire(b, s, st);
+        if (r != AUSM_OK) return r;
+      } else if (s->d.freeze_after > 0 && !s->frozen && s->steps >= s->d.freeze_after) {
+        r = freeze_solve(b, s);
         if (r != AUSM_OK) return r;
       }
     }

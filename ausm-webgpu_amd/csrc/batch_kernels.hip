@@ -58,11 +58,11 @@ __global__ __launch_bounds__(256) void batch_stage_kernel(const BatchArgs b) {
     const int tx = item.tile % S->tiles_x;
     const int tyb = item.tile / S->tiles_x;
     const float dt = fminf(P.maxdt, P.cfl / *S->maxws_cur);
-    // fast solves with van Albada (the reference's reconstruction) or first
-    // order (no reconstruction) take code with the limiter fixed at compile
-    // time
+    // fast solves with van Albada (the reference's reconstruction, limiter
+    // not frozen) or first order (no reconstruction) take code with the
+    // limiter fixed at compile time
     float ws;
-    if (FAST && a.limiter == LIM_VANALBADA)
+    if (FAST && a.limiter == LIM_VANALBADA && a.freeze == 0)
       ws = batch_tile<FAST, TY, COMP, LIM_VANALBADA, STAGE>(S->scheme, P, a, tx, tyb, s_face, dt);
     else if (FAST && a.limiter == LIM_NONE)
       ws = batch_tile<FAST, TY, COMP, LIM_NONE, STAGE>(S->scheme, P, a, tx, tyb, s_face, dt);

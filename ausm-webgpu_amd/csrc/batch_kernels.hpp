@@ -23,6 +23,8 @@ struct SolveDev {
   int n_ws;    // workgroup tiles of this solve = entries of ws_part
   int active;  // 0: slot not stepped
   int local_dt;  // 1: local time stepping
+  int freeze;    // 2: limiter frozen (psi recorded), else 0
+  float* psi;    // psi planes (solves that may freeze)
 };
 
 struct BatchItem {
@@ -63,6 +65,8 @@ __host__ __device__ inline StageArgs batch_args(const SolveDev& S, int stage, in
   a.local_dt = S.local_dt;
   a.acc = S.acc;
   a.lo = S.lo;
+  a.freeze = S.freeze;
+  a.psi = S.psi;
   a.fresh = 0;
   a.refresh = refresh;
   if (stage == 1) {

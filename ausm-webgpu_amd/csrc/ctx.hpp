@@ -56,6 +56,11 @@ struct ausm_ctx {
   // compensated FP32 state (AUSM_PREC_FP32C): q[0] holds hi
   float* acc = nullptr;  // increment accumulator planes
   float* lo = nullptr;   // low part of the solution
+  // limiter freezing: 2 while the record is in use, else 0; psi holds
+  // 2 kPsiPerFace planes of psi_plane(N, P) values of the current precision
+  // (sized for FP64, allocated on the first freeze)
+  int freeze = 0;
+  void* psi = nullptr;
 };
 
 namespace ausm {

@@ -131,6 +131,41 @@ __device__ __forceinline__ PrimT<R> pdiff(PrimT<R> a, PrimT<R> b) {
   return PrimT<R>{a.r - b.r, a.u - b.u, a.v - b.v, a.p - b.p};
 }
 
+// Limiter freezing, a build extension with no counterpart in the reference
+// (a convergence device for steady solves).  With r_L = (dL dF + eps) /
+// (dF^2 + eps), r_R = (dF dR + eps) / (dF^2 + eps), psi = phi(r) / r and
+// w = dF^2 / (dF^2 + eps), MUSCL's limited face values are exactly
+//   L = W_L + psi_L (w dL + (1 - w) dF) / 2,
+//   R = W_R - psi_R (w dR + (1 - w) dF) / 2
+// (phi(r) dF = psi r dF and r dF = w dL + (1 - w) dF).  Recording takes
+// psi = (r + 1) / (r^2 + 1) of van Albada (0 for r <= 0 with the clamped
+// form) and w of every face and primitive component from the current state,
+// in IEEE arithmetic in both math modes; a frozen face reconstructs with
+// the recorded psi and w, an operator that equals the limited one at the
+// recorded state and no longer switches.  Per face 12 values: psi_L of
+// (rho, u, v, p), psi_R, w.  mode 0: normal; 1: normal, and the 12 values
+// into rec; 2: from psi.
+template <typename R>
+struct FreezeT {
+  int mode;
+  const R* psi;  // 12 values (mode 2)
+  R* rec;        // 12 values (mode 1)
+};
+template <typename R>
+__device__ __forceinline__ R psi_of(int limiter, R r) {
+  if (limiter == LIM_VANALBADA_CLAMPED && !(r > R(0))) return R(0);
+  return (r + R(1)) / (r * r + R(1));
+}
+// the limiter ratios of the reference's MUSCL (ausm.js:244-245), IEEE, and
+// the weight w of the upwind slope
+template <typename R>
+__device__ __forceinline__ void muscl_ratios(R dL, R dF, R dR, R& rL, R& rR, R& w) {
+  const R eps = R(1e-6);
+  rL = (dL * dF + eps) / (dF * dF + eps);
+  rR = (dF * dR + eps) / (dF * dF + eps);
+  w = (dF * dF) / (dF * dF + eps);
+}
+
 // on the stencil differences dL = wl - wl2, dF = wr - wl, dR = wr2 - wr,
 // which a caller may share between neighbouring faces
 template <typename R>
@@ -155,20 +190,48 @@ __device__ __forceinline__ void muscl_exact(int limiter, R wl, R wr, R dL, R dF,
   }
 }
 
+// psi and w of the four components (limiter freezing, record)
+template <typename R>
+__device__ __forceinline__ void record_psi(int limiter, PrimT<R> dL, PrimT<R> dF, PrimT<R> dR,
+                                           R* rec) {
+  const R l[4] = {dL.r, dL.u, dL.v, dL.p}, f[4] = {dF.r, dF.u, dF.v, dF.p},
+          g[4] = {dR.r, dR.u, dR.v, dR.p};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    R rL, rR, w;
+    muscl_ratios(l[k], f[k], g[k], rL, rR, w);
+    rec[k] = psi_of(limiter, rL);
+    rec[4 + k] = psi_of(limiter, rR);
+    rec[8 + k] = w;
+  }
+}
+
 // b = W_L, c = W_R and the stencil differences dL = W_L - W_L2,
 // dF = W_R - W_L, dR = W_R2 - W_R
 template <int SCHEME, typename Real>
 __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P, int limiter, PrimT<Real> b,
                                                    PrimT<Real> c, PrimT<Real> dL, PrimT<Real> dF,
-                                                   PrimT<Real> dR, Real nx, Real ny) {
+                                                   PrimT<Real> dR, Real nx, Real ny,
+                                                   FreezeT<Real> frz = FreezeT<Real>{0, nullptr, nullptr}) {
   const Real gamma = P.gamma;
   const Real gammaMinus1 = gamma - Real(1.0);
   PrimT<Real> L = b, R = c;
-  if (limiter != LIM_NONE) {
+  if (limiter != LIM_NONE && frz.mode == 2) {
+    const Real* s = frz.psi;
+    auto fl = [&](Real w0, Real psL, Real psR, Real d0, Real df, Real d1, Real& l, Real& r) {
+      l = l + (Real(0.5) * psL) * (w0 * d0 + (Real(1) - w0) * df);
+      r = r - (Real(0.5) * psR) * (w0 * d1 + (Real(1) - w0) * df);
+    };
+    fl(s[8], s[0], s[4], dL.r, dF.r, dR.r, L.r, R.r);
+    fl(s[9], s[1], s[5], dL.u, dF.u, dR.u, L.u, R.u);
+    fl(s[10], s[2], s[6], dL.v, dF.v, dR.v, L.v, R.v);
+    fl(s[11], s[3], s[7], dL.p, dF.p, dR.p, L.p, R.p);
+  } else if (limiter != LIM_NONE) {
     muscl_exact(limiter, b.r, c.r, dL.r, dF.r, dR.r, L.r, R.r);
     muscl_exact(limiter, b.u, c.u, dL.u, dF.u, dR.u, L.u, R.u);
     muscl_exact(limiter, b.v, c.v, dL.v, dF.v, dR.v, L.v, R.v);
     muscl_exact(limiter, b.p, c.p, dL.p, dF.p, dR.p, L.p, R.p);
+    if (frz.mode == 1) record_psi(limiter, dL, dF, dR, frz.rec);
   }
   const Real u_L = L.u * nx + L.v * ny;
   const Real u_R = R.u * nx + R.v * ny;
@@ -310,14 +373,27 @@ __device__ __forceinline__ void muscl_fast(int limiter, float wl, float wr, floa
 template <int SCHEME>
 __device__ __forceinline__ Flux face_flux_fast(const DevParams& P, int limiter, Prim b, Prim c,
                                                Prim dL, Prim dF, Prim dR, float nx, float ny,
-                                               float len, float sx, float sy) {
+                                               float len, float sx, float sy,
+                                               FreezeT<float> frz = FreezeT<float>{0, nullptr, nullptr}) {
   const float gamma = P.gamma;
   Prim L = b, R = c;
-  if (limiter != LIM_NONE) {
+  if (limiter != LIM_NONE && frz.mode == 2) {
+    // w d + (1 - w) dF = dF + w (d - dF)
+    const float* s = frz.psi;
+    auto fl = [&](float w0, float psL, float psR, float d0, float df, float d1, float& l, float& r) {
+      l = fmaf(0.5f * psL, fmaf(w0, d0 - df, df), l);
+      r = fmaf(-0.5f * psR, fmaf(w0, d1 - df, df), r);
+    };
+    fl(s[8], s[0], s[4], dL.r, dF.r, dR.r, L.r, R.r);
+    fl(s[9], s[1], s[5], dL.u, dF.u, dR.u, L.u, R.u);
+    fl(s[10], s[2], s[6], dL.v, dF.v, dR.v, L.v, R.v);
+    fl(s[11], s[3], s[7], dL.p, dF.p, dR.p, L.p, R.p);
+  } else if (limiter != LIM_NONE) {
     muscl_fast(limiter, b.r, c.r, dL.r, dF.r, dR.r, L.r, R.r);
     muscl_fast(limiter, b.u, c.u, dL.u, dF.u, dR.u, L.u, R.u);
     muscl_fast(limiter, b.v, c.v, dL.v, dF.v, dR.v, L.v, R.v);
     muscl_fast(limiter, b.p, c.p, dL.p, dF.p, dR.p, L.p, R.p);
+    if (frz.mode == 1) record_psi(limiter, dL, dF, dR, frz.rec);
   }
   const float u_L = fmaf(L.u, nx, L.v * ny);
   const float u_R = fmaf(R.u, nx, R.v * ny);

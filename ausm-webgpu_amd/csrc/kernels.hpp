@@ -46,9 +46,23 @@ struct StageArgsT {
   // K1 + K2) and the low part of the solution, both SoA planes like qout
   R* acc;
   R* lo;
+  // limiter freezing (van Albada limiters, run-time limiter kernels only):
+  // 0 off; 1 record psi and w of every face (stage 0 only); 2 reconstruct
+  // with the record.  psi: 2 kPsiPerFace planes of psi_plane(N, P) values,
+  // the 12 (psi_L, psi_R, w of rho, u, v, p) of the vertical faces (row jf =
+  // face between rows jf-1 and jf, jf = 0..N), then the 12 of the
+  // horizontal faces (row r = left face of the cell, r = 0..N-1)
+  int freeze;
+  R* psi;
 };
 using StageArgs = StageArgsT<float>;
 using StageArgs64 = StageArgsT<double>;
+
+// values per face of a limiter-freezing record, and the plane stride
+constexpr int kPsiPerFace = 12;
+__host__ __device__ inline size_t psi_plane(int N, int P) {
+  return ((size_t)(N + 1) * (size_t)P + 63) / 64 * 64;
+}
 
 int stage_blocks(int M, int N, int ty);
 int stage_tiles_x(int M);
@@ -70,6 +84,11 @@ inline bool stage_args_ok(const StageArgsT<R>& a, bool comp) {
     if (a.stage >= 2 && !a.qn) return false;
     if (a.stage == 3 && a.refresh && !a.ws_out) return false;
     if (comp && (!a.acc || (a.stage == 3 && !a.lo))) return false;
+  }
+  if (a.freeze != 0) {
+    if (a.freeze != 1 && a.freeze != 2) return false;
+    if (!a.psi || (a.freeze == 1 && a.stage != 0)) return false;
+    if (a.limiter != LIM_VANALBADA && a.limiter != LIM_VANALBADA_CLAMPED) return false;
   }
   return true;
 }

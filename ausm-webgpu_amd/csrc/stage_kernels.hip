@@ -102,11 +102,11 @@ static hipError_t launch_ty_lim(const StageArgs& a, int ty, hipStream_t s) {
 
 // van Albada stages 1-3 (the stepping path of the reference's
 // reconstruction, fast and exact math) run code specialised on limiter and
-// stage; everything else (other limiters, residual evaluation) the general
-// kernel
+// stage; everything else (other limiters, residual evaluation, limiter
+// freezing) the general kernel
 template <int SCHEME, bool FAST>
 static hipError_t launch_ty(const StageArgs& a, int ty, hipStream_t s) {
-  if (a.limiter == LIM_VANALBADA) {
+  if (a.limiter == LIM_VANALBADA && a.freeze == 0) {
     switch (a.stage) {
       case 1: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
       case 2: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
@@ -151,7 +151,7 @@ static hipError_t launch_ty_comp_lim(const StageArgs& a, int ty, hipStream_t s) 
 template <int SCHEME, bool FAST>
 static hipError_t launch_ty_comp(const StageArgs& a, int ty, hipStream_t s) {
   if constexpr (FAST) {
-    if (a.limiter == LIM_VANALBADA) {
+    if (a.limiter == LIM_VANALBADA && a.freeze == 0) {
       switch (a.stage) {
         case 1: return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
         case 2: return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
@@ -191,7 +191,7 @@ static hipError_t launch_ty64_lim(const StageArgs64& a, int ty, hipStream_t s) {
 // van Albada stages 1-3 specialised as in launch_ty
 template <int SCHEME>
 static hipError_t launch_ty64(const StageArgs64& a, int ty, hipStream_t s) {
-  if (a.limiter == LIM_VANALBADA) {
+  if (a.limiter == LIM_VANALBADA && a.freeze == 0) {
     switch (a.stage) {
       case 1: return launch_ty64_lim<SCHEME, LIM_VANALBADA, 1>(a, ty, s);
       case 2: return launch_ty64_lim<SCHEME, LIM_VANALBADA, 2>(a, ty, s);

@@ -97,11 +97,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int SCHEME, bool FAST, typename R>
 __device__ __forceinline__ FluxT<R> face_flux(const DevParamsT<R>& P, int lim, PrimT<R> b,
                                               PrimT<R> c, PrimT<R> dL, PrimT<R> dF, PrimT<R> dR,
-                                              R dx, R dy, R nx, R ny, R len) {
+                                              R dx, R dy, R nx, R ny, R len,
+                                              FreezeT<R> frz = FreezeT<R>{0, nullptr, nullptr}) {
   if constexpr (FAST) {
-    return face_flux_fast<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny, len, -dy, dx);
+    return face_flux_fast<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny, len, -dy, dx, frz);
   } else {
-    const FluxT<R> G = face_flux_exact<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny);
+    const FluxT<R> G = face_flux_exact<SCHEME>(P, lim, b, c, dL, dF, dR, nx, ny, frz);
     return FluxT<R>{G.f0 * len, G.f1 * len, G.f2 * len, G.f3 * len};
   }
 }
@@ -340,12 +341,40 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
 #pragma unroll
     for (int k = 1; k < NR; ++k)
       if (k > 1 || rb == 0) dd[k] = pdiff(w[k], w[k - 1]);
-    // vertical face over slots k-2..k+1 with edge (ex, ey), times its length
+    // limiter freezing (kernels with the run-time limiter only): a frozen
+    // face loads its 12 recorded values where it is evaluated, and a
+    // recording pass stores them at the lanes that own the column.  Planes
+    // 0-11: vertical faces, row jf = face between rows jf-1 and jf; planes
+    // 12-23: horizontal faces, row r = left face of the cell.
+    constexpr bool FRZ = LIM < 0;
+    const int frz = FRZ ? a.freeze : 0;
+    const size_t FPl = psi_plane(N, Pp);
+    auto get_psi = [&](size_t plane0, int row, R* ps) {
+      const R* p = a.psi + plane0 + (size_t)row * (size_t)Pp;
+#pragma unroll
+      for (int c = 0; c < kPsiPerFace; ++c) ps[c] = ld_row(p + c * FPl, bc);
+    };
+    auto put_psi = [&](size_t plane0, int row, const R* rec) {
+      R* p = a.psi + plane0 + (size_t)row * (size_t)Pp;
+#pragma unroll
+      for (int c = 0; c < kPsiPerFace; ++c) st_row(p + c * FPl, bc, rec[c]);
+    };
+    // vertical face over slots k-2..k+1 (face index j0-2+k) with edge
+    // (ex, ey), times its length
     auto vface = [&](int k, R ex, R ey, R& len) -> Flux {
       R nx, ny;
       face_normal<FAST>(ex, ey, nx, ny, len);
-      return face_flux<SCHEME, FAST>(P, lim, w[k - 1], w[k], dd[k - 1], dd[k], dd[k + 1], ex, ey, nx,
-                                     ny, len);
+      if constexpr (FRZ) {
+        R ps[kPsiPerFace];
+        if (frz == 2) get_psi(0, min(j0 - 2 + k, N), ps);
+        const Flux G = face_flux<SCHEME, FAST>(P, lim, w[k - 1], w[k], dd[k - 1], dd[k], dd[k + 1],
+                                               ex, ey, nx, ny, len, FreezeT<R>{frz, ps, ps});
+        if (frz == 1 && owned) put_psi(0, j0 - 2 + k, ps);
+        return G;
+      } else {
+        return face_flux<SCHEME, FAST>(P, lim, w[k - 1], w[k], dd[k - 1], dd[k], dd[k + 1], ex, ey,
+                                       nx, ny, len);
+      }
     };
 
     // top face of a full tile (face index j0 + TY, window slots TY..TY+3),
@@ -400,8 +429,17 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         R nxX, nyX, lL;
         const R ex = vx1 - vx0, ey = vy1 - vy0;
         face_normal<FAST>(ex, ey, nxX, nyX, lL);
-        const Flux PX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), ex,
-                                                ey, nxX, nyX, lL);
+        Flux PX;
+        if constexpr (FRZ) {
+          R ps[kPsiPerFace];
+          if (frz == 2) get_psi(kPsiPerFace * FPl, r, ps);
+          PX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), ex, ey, nxX,
+                                       nyX, lL, FreezeT<R>{frz, ps, ps});
+          if (frz == 1 && owned) put_psi(kPsiPerFace * FPl, r, ps);
+        } else {
+          PX = face_flux<SCHEME, FAST>(P, lim, wl, w0, from_left(dx), dx, from_right(dx), ex, ey, nxX,
+                                       nyX, lL);
+        }
 
         // vertical face at the top of cell (i, r): stencil r-1..r+2; the last
         // row of a tile (reached only when the tile is full) has it already

@@ -185,6 +185,22 @@ int ausm_set_dt_mode(ausm_ctx* ctx, int mode);
 /* rows per wavefront tile of the stage kernel (0 = autotuned) */
 int ausm_set_tile(ausm_ctx* ctx, int rows_per_wave);
 
+/* Limiter freezing.  No counterpart in the reference (a convergence device
+ * for steady solves with a MUSCL reconstruction, whose limiter switching
+ * can hold the residual in a limit cycle).  With r the reference's limiter
+ * ratio, psi = phi(r)/r and w = dF^2 / (dF^2 + 1e-6), the limited face
+ * values are exactly W_L + psi_L (w dL + (1 - w) dF)/2 and
+ * W_R - psi_R (w dR + (1 - w) dF)/2.  on = 1 records psi (van Albada:
+ * AUSM_VANALBADA, or AUSM_VANALBADA_CLAMPED with psi = 0 for r <= 0) and w
+ * of every face and primitive component from the current state; every
+ * later stage, residual and norm reconstructs with the record: the same
+ * operator at the recorded state, with the limiter no longer switching.
+ * on = 0, a new state, grid, limiter or precision ends it.  Needs a state;
+ * other limiters: AUSM_ERR_ARG.  Device memory: 24 planes of
+ * (N+1) x round_up(M, 64) doubles, allocated on first use. */
+int ausm_freeze_limiter(ausm_ctx* ctx, int on);
+int ausm_limiter_frozen(ausm_ctx* ctx, int* on);
+
 /* Arithmetic precision of the solve.  No counterpart in the reference,
  * whose storage and arithmetic are all f32 (FP32 here).  FP64 runs the same
  * scheme and pass structure in double (exact arithmetic, FP64 state, the
@@ -301,6 +317,9 @@ typedef struct ausm_solve_desc {
   double res_drop;       /* converged when res_l2 <= res_drop * first res_l2 (0 = never) */
   double ref_length;     /* for Cl/Cd */
   int dt_mode;           /* AUSM_DT_GLOBAL or AUSM_DT_LOCAL (ausm_set_dt_mode) */
+  int freeze_after;      /* van Albada limiters: freeze the limiter (ausm_freeze_limiter)
+                            at the first convergence check at or after this many
+                            steps; 0 = never */
 } ausm_solve_desc;
 
 typedef struct ausm_solve_status {

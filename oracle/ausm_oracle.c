@@ -109,12 +109,38 @@ static inline R minmod2(R a, R b) {
 /* face flux: fluxInterfaceStateCode + scheme bodies (ausm.js:128-398)       */
 /* ------------------------------------------------------------------------ */
 
+/* Limiter freezing: a build extension with no counterpart in the reference
+ * (a convergence device for steady MUSCL solves).  With the reference's
+ * ratios r_L = (dL dF + eps) / (dF^2 + eps), r_R = (dF dR + eps) /
+ * (dF^2 + eps), psi = phi(r)/r and w = dF^2 / (dF^2 + eps), the limited face
+ * values are exactly L = W_L + psi_L (w dL + (1 - w) dF) / 2 and
+ * R = W_R - psi_R (w dR + (1 - w) dF) / 2, since phi(r) dF = psi r dF and
+ * r dF = w dL + (1 - w) dF.  Recording (mode 1) evaluates the limited flux
+ * as usual and stores, per face and primitive component, psi = (r + 1) /
+ * (r^2 + 1) of van Albada (0 for r <= 0 with the clamped form) and w; a
+ * frozen face (mode 2) reconstructs with the stored values.  set_freeze
+ * selects the mode and the record for every later flux pass: vertical faces
+ * (N+1) x M x 12, then horizontal faces N x M x 12, each face's 12 values
+ * psi_L of (rho, u, v, p), psi_R, w.  State of the library, not thread-safe:
+ * callers set it around each call. */
+static int FN(frz_mode) = 0;
+static R* FN(frz_psi) = NULL;
+void FN(set_freeze)(int mode, R* psi) {
+  FN(frz_mode) = psi ? mode : 0;
+  FN(frz_psi) = psi;
+}
+static inline R psi_of(int limiter, R r) {
+  if (limiter == LIM_VANALBADA_CLAMPED && !(r > (R)0)) return (R)0;
+  return (r + (R)1) / (r * r + (R)1);
+}
+
 /* One face flux from the four stencil states QL2, QL, QR, QR2 (conserved)
  * and the face unit normal n.  Writes the per-unit-length flux to F[4].
  * Follows ausm.js:183-267 (interface states), 271-336 (AUSM+-up) and
- * 345-393 (SLAU / SLAU2). */
-void FN(face_flux)(const R* par, int scheme, int limiter, const R* QL2, const R* QL,
-                   const R* QR, const R* QR2, R nx, R ny, R* F) {
+ * 345-393 (SLAU / SLAU2).  frz, psi_in, psi_out: limiter freezing (above). */
+static void face_flux_frz(const R* par, int scheme, int limiter, const R* QL2, const R* QL,
+                          const R* QR, const R* QR2, R nx, R ny, R* F, int frz,
+                          const R* psi_in, R* psi_out) {
   const R gamma = par[P_GAMMA];
   const R gammaMinus1 = gamma - (R)1;
   const R eps = (R)1e-6;
@@ -131,7 +157,11 @@ void FN(face_flux)(const R* par, int scheme, int limiter, const R* QL2, const R*
       R dL = WL[c] - WL2[c];
       R dF = WR[c] - WL[c];
       R dR = WR2[c] - WR[c];
-      if (limiter == LIM_MINMOD) {
+      if (frz == 2) {
+        const R w = psi_in[8 + c];
+        L[c] = L[c] + ((R)0.5 * psi_in[c]) * (w * dL + ((R)1 - w) * dF);
+        Rr[c] = Rr[c] - ((R)0.5 * psi_in[4 + c]) * (w * dR + ((R)1 - w) * dF);
+      } else if (limiter == LIM_MINMOD) {
         /* two-slope minmod (ausm.js:164-166), wired as a slope limiter */
         L[c] = L[c] + (R)0.5 * minmod2(dL, dF);
         Rr[c] = Rr[c] - (R)0.5 * minmod2(dF, dR);
@@ -148,6 +178,11 @@ void FN(face_flux)(const R* par, int scheme, int limiter, const R* QL2, const R*
         }
         L[c] = L[c] + ((R)0.5 * phiL) * dF;
         Rr[c] = Rr[c] - ((R)0.5 * phiR) * dF;
+        if (frz == 1) {
+          psi_out[c] = psi_of(limiter, rL);
+          psi_out[4 + c] = psi_of(limiter, rR);
+          psi_out[8 + c] = (dF * dF) / (dF * dF + eps);
+        }
       }
     }
   }
@@ -249,6 +284,11 @@ void FN(face_flux)(const R* par, int scheme, int limiter, const R* QL2, const R*
   F[3] = ph3 * mdot + (R)0;
 }
 
+void FN(face_flux)(const R* par, int scheme, int limiter, const R* QL2, const R* QL,
+                   const R* QR, const R* QR2, R nx, R ny, R* F) {
+  face_flux_frz(par, scheme, limiter, QL2, QL, QR, QR2, nx, ny, F, 0, NULL, NULL);
+}
+
 /* face unit normal from two vertices: normalize then rotate +90 degrees */
 static inline void face_normal(R dx, R dy, R* nx, R* ny) {
   R len = rsqrt_(dx * dx + dy * dy);
@@ -263,6 +303,8 @@ static inline void face_normal(R dx, R dy, R* nx, R* ny) {
  * left zero (the reference computes it but never reads it). */
 void FN(flux_pass)(const R* par, int scheme, int limiter, int vertical, int M, int N,
                    const R* vtx, const R* st, R* flux) {
+  const int frz = limiter == LIM_NONE ? 0 : FN(frz_mode);
+  R* const psi = FN(frz_psi);
   if (vertical) {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y <= N; ++y)
@@ -271,9 +313,10 @@ void FN(flux_pass)(const R* par, int scheme, int limiter, int vertical, int M, i
         const R* v2 = vtx + ((size_t)y * M + (x + 1) % M) * 2;
         R nx, ny;
         face_normal(v1[0] - v2[0], v1[1] - v2[1], &nx, &ny);
-        FN(face_flux)(par, scheme, limiter, load_state(st, M, x, y - 2), load_state(st, M, x, y - 1),
+        R* ps = frz ? psi + ((size_t)y * M + x) * 12 : NULL;
+        face_flux_frz(par, scheme, limiter, load_state(st, M, x, y - 2), load_state(st, M, x, y - 1),
                       load_state(st, M, x, y), load_state(st, M, x, y + 1), nx, ny,
-                      flux + ((size_t)y * M + x) * 4);
+                      flux + ((size_t)y * M + x) * 4, frz, ps, ps);
       }
   } else {
     const int W = M + 1;
@@ -284,9 +327,10 @@ void FN(flux_pass)(const R* par, int scheme, int limiter, int vertical, int M, i
         const R* v2 = vtx + ((size_t)(y + 1) * M + x) * 2;
         R nx, ny;
         face_normal(v2[0] - v1[0], v2[1] - v1[1], &nx, &ny);
-        FN(face_flux)(par, scheme, limiter, load_state(st, M, x - 2, y), load_state(st, M, x - 1, y),
+        R* ps = frz ? psi + ((size_t)(N + 1) * M + (size_t)y * M + x) * 12 : NULL;
+        face_flux_frz(par, scheme, limiter, load_state(st, M, x - 2, y), load_state(st, M, x - 1, y),
                       load_state(st, M, x, y), load_state(st, M, x + 1, y), nx, ny,
-                      flux + ((size_t)y * W + x) * 4);
+                      flux + ((size_t)y * W + x) * 4, frz, ps, ps);
       }
       for (int c = 0; c < 4; ++c) flux[((size_t)y * W + M) * 4 + c] = (R)0;
     }

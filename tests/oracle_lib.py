@@ -4,6 +4,7 @@ The oracle is a plain-C restatement of the reference solver (ref32 = FP32,
 ref64 = FP64).  Tests use it as the checker for the HIP library; nothing in
 the product imports this module.
 """
+import contextlib
 import ctypes
 import math
 import os
@@ -92,6 +93,7 @@ def _declare(L, prefixes=(("orc32_", ctypes.c_float), ("orc64_", ctypes.c_double
         g("derived_field").argtypes = [vp, ci, ci, vp, ci, vp, vp]
         g("residual_norm").argtypes = [ci, ci, vp, ci]
         g("residual_norm").restype = cd
+        g("set_freeze").argtypes = [ci, vp]
     if not geometry:
         return
     for n in ("geo_naca4_point",):
@@ -238,6 +240,21 @@ class Oracle:
     def arr(self, a):
         return np.ascontiguousarray(a, dtype=self.dt)
 
+    @contextlib.contextmanager
+    def freeze(self, mode, psi):
+        """Limiter freezing for the flux passes inside the block: mode 1
+        records psi into `psi`, mode 2 reconstructs with it.  psi: a flat
+        array of (2N+1) x M x 12 values of this precision."""
+        assert psi.dtype == self.dt and psi.flags.c_contiguous
+        self.f("set_freeze")(mode, _ptr(psi))
+        try:
+            yield
+        finally:
+            self.f("set_freeze")(0, None)
+
+    def psi_array(self, M, N):
+        return np.zeros((2 * N + 1) * M * 12, dtype=self.dt)
+
     # grid -------------------------------------------------------------
     def grid(self, body, outer, N, iters):
         """Interpolation + Jacobi TTM, in this precision.  Returns (N+1,M,2)."""
@@ -360,6 +377,7 @@ class Solver:
         self.maxws[0] = self.o.max_ws(self.par, self.s2, self.area, self.flen)
         self.last_dt = None
         self.lo = np.zeros_like(self.s2)
+        self.psi = None  # recorded limiter (freeze_limiter)
 
     def set_state(self, st):
         st = self.o.arr(st)
@@ -367,9 +385,37 @@ class Solver:
         self.s1[...] = st
         self.s2[...] = st
         self.lo[...] = 0
+        self.psi = None  # a new state ends limiter freezing
         self.maxws[0] = self.o.max_ws(self.par, self.s2, self.area, self.flen)
 
+    def freeze_limiter(self, on=True):
+        """Record psi from the current state (a residual pass) and step with
+        the frozen reconstruction from now on; on=False ends it."""
+        if not on:
+            self.psi = None
+            return
+        psi = self.o.psi_array(self.M, self.N)
+        with self.o.freeze(1, psi):
+            self.o.residual(self.par, self.scheme, self.limiter, self.vtx, self.s2, self.area,
+                            self.flen)
+        self.psi = psi
+
+    def residual(self):
+        """L(Q) of the current state (frozen if the limiter is)."""
+        with self._frozen():
+            return self.o.residual(self.par, self.scheme, self.limiter, self.vtx, self.s2,
+                                   self.area, self.flen)
+
+    def _frozen(self):
+        if self.psi is None:
+            return contextlib.nullcontext()
+        return self.o.freeze(2, self.psi)
+
     def step(self, n, want_res=False):
+        with self._frozen():
+            return self._step(n, want_res)
+
+    def _step(self, n, want_res=False):
         if self.comp:
             assert not want_res
             self.last_dt = self.o.f("run_steps_comp")(
