@@ -14,6 +14,12 @@ CFL step) and the compensated FP32 state: with the reference's single
 global step the far field of a fine O-grid barely moves, and plain FP32
 accumulation stagnates at a residual drop of ~1e-5 with Cl off by ~1e-4
 or more.  Both are per-run options (dt_mode, precision).
+
+A request may carry "freeze_after" (steps): its van Albada limiter is
+frozen (Solver.freeze_limiter) at the first convergence check at or after
+that many steps, in the batched service, the one-by-one baseline and the
+FP64 re-solve alike, so that a MUSCL solve held in a limit cycle by its
+limiter can still converge.
 """
 import math
 import time
@@ -90,7 +96,8 @@ def run_batched(reqs, arrivals=None, check_every=100, max_solves=64, hbm_budget_
             sids[r["id"]] = b.submit(r["vertices"], freestream(r["mach"], r["aoa"]),
                                      scheme=r["scheme"], limiter=r.get("limiter", 0),
                                      cfl=r["cfl"], max_steps=r["max_steps"],
-                                     res_drop=r["res_drop"], ref_length=1.0, dt_mode=dt_mode)
+                                     res_drop=r["res_drop"], ref_length=1.0, dt_mode=dt_mode,
+                                     freeze_after=r.get("freeze_after", 0))
             nxt += 1
         active = [k for k in sids if k not in done]
         if not active:
@@ -141,6 +148,7 @@ def run_sequential(reqs, arrivals=None, check_every=100, math="fast", device=0, 
         s.prepare_state(True)
         res0 = s.residual_norm()
         steps, state, res = 0, "running", res0
+        fa, frozen = r.get("freeze_after", 0), False
         while state == "running":
             try:
                 info = s.step(check_every)
@@ -155,6 +163,9 @@ def run_sequential(reqs, arrivals=None, check_every=100, math="fast", device=0, 
                 state = "converged"
             elif steps >= r["max_steps"]:
                 state = "capped"
+            elif fa > 0 and not frozen and steps >= fa:
+                s.freeze_limiter()
+                frozen = True
         st = {"state": state, "steps": steps, "res_l2": res, "res_l2_0": res0}
         if state != "failed":
             f = s.forces(1.0)
@@ -194,12 +205,19 @@ def accuracy_vs_fp64(reqs, done, n_check=4, check_every=100, dt_mode="local", de
         res0 = s.residual_norm()
         steps, res = 0, res0
         at_criterion = None
+        fa, frozen = r.get("freeze_after", 0), False
         while steps < step_factor * r["max_steps"] and res > deeper * r["res_drop"] * res0:
             res = s.step(check_every)["res_l2"]
             steps += check_every
-            if at_criterion is None and res <= r["res_drop"] * res0:
+            met_now = at_criterion is None and res <= r["res_drop"] * res0
+            if met_now:
                 f = s.forces(1.0)
                 at_criterion = (steps, f["cl"], f["cd"])
+            # frozen at the check the served solve freezes at (one where it
+            # does not retire), or later on the way to the deeper answer
+            if fa > 0 and not frozen and steps >= fa and not met_now:
+                s.freeze_limiter()
+                frozen = True
         f = s.forces(1.0)
         s.close()
         c = {"id": r["id"], "M": r["M"], "N": r["N"], "mach": r["mach"], "steps": st["steps"],

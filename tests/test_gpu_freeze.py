@@ -231,3 +231,27 @@ def test_freezing_converges_a_stalled_transonic_solve():
     assert drop <= 1e-5
     assert abs(f["cl"] - np.mean(cls)) < 2e-3 and abs(f["cd"] - np.mean(cds)) < 2e-3
     s.close()
+
+
+def test_served_freezing_equals_one_by_one():
+    """The serving loop freezes a request's limiter at the same check in the
+    batched service and in the one-by-one baseline: 6 requests of 128 x 64
+    cells (drawn as the serving mix, MUSCL van Albada), frozen after 300
+    steps, 1000-step cap -- state, steps, Cl and Cd agree exactly,
+    whatever each request's outcome."""
+    from ausm import serve
+    reqs = serve.make_requests(6, seed=3, sizes=[(128, 64)], mach_range=(0.3, 3.0),
+                               max_steps=1000, res_drop=1e-5)
+    for r in reqs:
+        r["limiter"] = VANALBADA
+        r["freeze_after"] = 300
+    serve.make_grids(reqs, iters=500)
+    a = serve.run_batched(reqs, None, check_every=100, max_solves=8, dt_mode="local",
+                          precision="fp32c")
+    b = serve.run_sequential(reqs, None, check_every=100, dt_mode="local", precision="fp32c")
+    for r in reqs:
+        x, y = a["results"][r["id"]], b["results"][r["id"]]
+        print(f"[freeze] served request {r['id']} Mach {r['mach']:.2f}: {x['state']} after "
+              f"{x['steps']} steps, drop {x['res_l2'] / x['res_l2_0']:.1e}", flush=True)
+        assert (x["state"], x["steps"], x.get("cl"), x.get("cd")) == \
+            (y["state"], y["steps"], y.get("cl"), y.get("cd")), r["id"]
