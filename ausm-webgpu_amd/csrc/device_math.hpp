@@ -55,6 +55,55 @@ using Prim = PrimT<float>;
 using Cons = ConsT<float>;
 using Flux = FluxT<float>;
 
+// ---------------------------------------------------------------------------
+// IEEE FP32 division in fewer instructions, for operands in a known range.
+// The compiler's correctly rounded division is v_div_scale (x2), v_rcp_f32,
+// a Newton step on the reciprocal, a quotient with two FMA-residual
+// corrections, v_div_fmas and v_div_fixup: 11 VALU instructions, of which
+// the two scalings and the fixup only act on operands near the ends of the
+// exponent range (denormal or huge divisor, numerator below 2^-103, quotient
+// near overflow or underflow) and on zeros, infinities and NaNs.  div_rn is
+// the same sequence without them (8 instructions):
+//   y = rcp(d);  y += y (1 - d y);  q = n y;  q += y (n - d q);  q += y (n - d q)
+// and returns the IEEE quotient, bit for bit, whenever d > 0 is normal with
+// d < 2^126, n = +0 or 2^-100 <= |n|, and |n / d| < 2^96: the residuals
+// n - d q are then exact, and two corrections of a quotient within an ulp,
+// with a reciprocal within an ulp, round correctly (Markstein).  div_by
+// takes the refined reciprocal recip_rn(d), so that quotients with one
+// divisor share it.  Every call site states why its operands are in range;
+// elsewhere the exact path keeps the compiler's division (quotients whose
+// numerators can be arbitrarily small, such as momenta and velocities at
+// the front of a disturbance spreading into a uniform flow).  A zero,
+// infinite or NaN divisor or an infinite numerator gives NaN where IEEE
+// may give an infinity: a state that reaches them is non-finite either
+// way.  R = double (the FP64 solve) always divides with the compiler's
+// sequence.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R recip_rn(R d) {
+  if constexpr (sizeof(R) == 4) {
+    const float y = __builtin_amdgcn_rcpf(d);
+    return fmaf(fmaf(-d, y, 1.0f), y, y);
+  } else {
+    return R(1) / d;
+  }
+}
+template <typename R>
+__device__ __forceinline__ R div_by(R n, R d, R y) {
+  if constexpr (sizeof(R) == 4) {
+    float q = n * y;
+    q = fmaf(fmaf(-d, q, n), y, q);
+    return fmaf(fmaf(-d, q, n), y, q);
+  } else {
+    (void)y;
+    return n / d;
+  }
+}
+template <typename R>
+__device__ __forceinline__ R div_rn(R n, R d) {
+  return div_by(n, d, recip_rn(d));
+}
+
 template <typename R>
 __device__ __forceinline__ R wstep(R edge, R x) { return x >= edge ? R(1) : R(0); }
 template <typename R>
@@ -109,15 +158,21 @@ __device__ __forceinline__ void normal_fast(float dx, float dy, float& nx, float
 // ---------------------------------------------------------------------------
 // Exact face flux (ausm.js:183-393)
 // ---------------------------------------------------------------------------
+// The limiter functions take the MUSCL ratio r below, which is +0 or
+// 2^-72 <= |r| <= 2^50 for any finite state (numerator +0 or at least
+// 2^-44 in magnitude, divisor in [1e-6, ~1e8]): the numerators r^2 + r and
+// r + |r| are then +0 or at least 2^-72 in magnitude (r^2 + r near r = -1
+// is a multiple of 2^-48) and the divisors lie in [1, 2^101], so div_rn is
+// the IEEE quotient.
 template <typename R>
 __device__ __forceinline__ R van_albada_exact(R r) {
   const R r2 = r * r;
-  return (r2 + r) / (r2 + R(1.0));
+  return div_rn(r2 + r, r2 + R(1.0));
 }
 template <typename R>
 __device__ __forceinline__ R van_leer_exact(R r) {
   const R a = fabs(r);
-  return (r + a) / (R(1.0) + a);
+  return div_rn(r + a, R(1.0) + a);
 }
 template <typename R>
 __device__ __forceinline__ R minmod_exact(R a, R b) {
@@ -176,8 +231,16 @@ __device__ __forceinline__ void muscl_exact(int limiter, R wl, R wr, R dL, R dF,
     L = wl + R(0.5) * minmod_exact(dL, dF);
     Rr = wr - R(0.5) * minmod_exact(dF, dR);
   } else {
-    const R rL = (dL * dF + eps) / (dF * dF + eps);
-    const R rR = (dF * dR + eps) / (dF * dF + eps);
+    // the ratios share their divisor dF^2 + eps, at least 1e-6 and at most
+    // ~1e8 for primitive differences of any physical state; each numerator
+    // x + eps is +0 (exact cancellation) or at least 2^-44 in magnitude
+    // (nearly cancelling, x + eps is a multiple of the smaller ulp of its
+    // two terms; otherwise it is near eps or near x), so div_by is the
+    // IEEE quotient
+    const R den = dF * dF + eps;
+    const R yd = recip_rn(den);
+    const R rL = div_by(dL * dF + eps, den, yd);
+    const R rR = div_by(dF * dR + eps, den, yd);
     R phiL = limiter == LIM_VANLEER ? van_leer_exact(rL) : van_albada_exact(rL);
     R phiR = limiter == LIM_VANLEER ? van_leer_exact(rR) : van_albada_exact(rR);
     if (limiter == LIM_VANALBADA_CLAMPED) {
@@ -237,17 +300,22 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
   const Real u_R = R.u * nx + R.v * ny;
   const Real vvL = L.u * L.u + L.v * L.v;
   const Real vvR = R.u * R.u + R.v * R.v;
-  const Real h_L = (L.p / (gammaMinus1 * L.r) + Real(0.5) * vvL) + L.p / L.r;
-  const Real h_R = (R.p / (gammaMinus1 * R.r) + Real(0.5) * vvR) + R.p / R.r;
+  // p / ((gamma-1) rho) and p / rho: the reconstructed pressure is
+  // W.p >= 1e-7 plus a rounded product, so it is +0 or at least ~1e-15 in
+  // magnitude, and rho of a physical state is far inside div_rn's range
+  const Real h_L = (div_rn(L.p, gammaMinus1 * L.r) + Real(0.5) * vvL) + div_rn(L.p, L.r);
+  const Real h_R = (div_rn(R.p, gammaMinus1 * R.r) + Real(0.5) * vvR) + div_rn(R.p, R.r);
   Real mdot, P_i;
   if constexpr (SCHEME == SCHEME_AUSMUP) {
     // ausm.js:273-327
-    const Real M_inf2 = (P.vx * P.vx + P.vy * P.vy) / ((gamma * P.inP) / L.r);
+    // free-stream numerators (|V|^2 is +0 or a normal number, gamma p_inf > 0)
+    const Real M_inf2 = div_rn(P.vx * P.vx + P.vy * P.vy, div_rn(gamma * P.inP, L.r));
     const Real sigma = Real(1.0);
     const Real h_t = Real(0.5) * (h_L + h_R);
     const Real aStar = sqrt(fmax(Real(1e-6), ((Real(2.0) * gammaMinus1) / (gamma + Real(1.0))) * h_t));
-    const Real aHat_L = (aStar * aStar) / fmax(u_L, aStar);
-    const Real aHat_R = (aStar * aStar) / fmax(-u_R, aStar);
+    // aStar >= 1e-3, and the divisors are at least aStar
+    const Real aHat_L = div_rn(aStar * aStar, fmax(u_L, aStar));
+    const Real aHat_R = div_rn(aStar * aStar, fmax(-u_R, aStar));
     const Real a_i = fmin(aHat_L, aHat_R);
     const Real M_L = u_L / a_i;
     const Real M_R = u_R / a_i;
@@ -285,8 +353,9 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     mdot = (a_i * M_i) * (M_i > Real(0.0) ? L.r : R.r);
   } else {
     // ausm.js:348-384
-    const Real a_L = sqrt((gamma * fmax(L.p, Real(1e-10))) / fmax(L.r, Real(1e-10)));
-    const Real a_R = sqrt((gamma * fmax(R.p, Real(1e-10))) / fmax(R.r, Real(1e-10)));
+    // numerators at least 1.4e-10, divisors at least 1e-10
+    const Real a_L = sqrt(div_rn(gamma * fmax(L.p, Real(1e-10)), fmax(L.r, Real(1e-10))));
+    const Real a_R = sqrt(div_rn(gamma * fmax(R.p, Real(1e-10)), fmax(R.r, Real(1e-10))));
     const Real a_i = (a_L + a_R) * Real(0.5);
     const Real M_L = u_L / a_i;
     const Real M_R = u_R / a_i;
@@ -313,7 +382,8 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     const Real g = wsat(-M_L) * wsat(M_R);
     const Real ubp = wmix(ubar_n, fabs(u_L), g);
     const Real ubm = wmix(ubar_n, fabs(u_R), g);
-    mdot = Real(0.5) * ((L.r * (u_L + ubp) + R.r * (u_R - ubm)) * (Real(1.0) - g) - (X / a_i) * delta_p);
+    // X = (1 - Marc)^2 with Marc in [0, 1] is +0 or at least 2^-48
+    mdot = Real(0.5) * ((L.r * (u_L + ubp) + R.r * (u_R - ubm)) * (Real(1.0) - g) - div_rn(X, a_i) * delta_p);
   }
   const bool left = mdot > Real(0.0);
   FluxT<Real> F;

@@ -130,10 +130,12 @@ __device__ __forceinline__ R cell_wave_speed(const DevParamsT<R>& P, R rho, R mu
     const R vx = mu / rho, vy = mv / rho;
     const R vv = vx * vx + vy * vy;
     const R p = (e - (R(0.5) * rho) * vv) * (P.gamma - R(1));
-    const R T = p / rho;
+    // p and rho of a physical state, and (|u| + c) S >= c S > 0 over twice
+    // the cell area, are inside div_rn's range
+    const R T = div_rn(p, rho);
     const R c = sqrt(P.gamma * T);
     const R speed = sqrt(vv);
-    return ((speed + c) * S) / (R(2) * area);
+    return div_rn((speed + c) * S, R(2) * area);
   }
 }
 

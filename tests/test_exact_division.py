@@ -1,0 +1,153 @@
+"""The exact path's short IEEE division (div_rn / div_by / recip_rn in
+device_math.hpp), checked in exact rational arithmetic.
+
+The device sequence is
+    y0 = v_rcp_f32(d)                  (within 1 ulp of 1/d)
+    y  = fma(fma(-d, y0, 1), y0, y0)
+    q0 = n * y
+    q1 = fma(fma(-d, q0, n), y, q0)
+    q  = fma(fma(-d, q1, n), y, q1)
+with every operation rounded once to FP32 (round to nearest, ties to even).
+It claims the IEEE quotient RN(n / d) bit for bit for d > 0 normal with
+d < 2^126, n = +0 or 2^-100 <= |n|, |n / d| < 2^96.  Here each operation is
+evaluated exactly and rounded to FP32 by hand, for every reciprocal the
+hardware may return (each FP32 value within 1 ulp of 1/d), over random
+operands spanning the claimed range and over quotients placed next to a
+rounding midpoint (constructed to within ~2^-48 of it: the cases on which
+the sequence without its Newton step rounds the wrong way, checked below
+too, so that the test can fail).  The end-to-end bitwise parity tests
+of the exact path cover the operands the solver actually divides.
+"""
+import math
+import random
+from fractions import Fraction
+
+import pytest
+
+P = 24  # FP32 significand bits
+
+
+def rn32(x):
+    """Round a rational to the nearest FP32 value (ties to even), with
+    gradual underflow (the device keeps FP32 denormals)."""
+    if x == 0:
+        return Fraction(0)
+    s = -1 if x < 0 else 1
+    a = abs(x)
+    e = math.floor(math.log2(a.numerator) - math.log2(a.denominator))
+    # fix e so that 2^e <= a < 2^(e+1)
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    assert e < 128, e
+    ulp = Fraction(2) ** (max(e, -126) - P + 1)
+    m = a / ulp
+    f = m.numerator // m.denominator
+    rem = m - f
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and f % 2 == 1):
+        f += 1
+    return s * f * ulp
+
+
+def ulp32(x):
+    a = abs(x)
+    e = math.floor(math.log2(a))
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    return Fraction(2) ** (e - P + 1)
+
+
+def fma(a, b, c):
+    return rn32(a * b + c)
+
+
+def seq(n, d, y0, newton=True):
+    y = fma(fma(-d, y0, 1), y0, y0) if newton else y0
+    q = rn32(n * y)
+    for _ in range(2):
+        q = fma(fma(-d, q, n), y, q)
+    return q
+
+
+def recips(d):
+    """Every FP32 value within one ulp of 1/d."""
+    x = 1 / d
+    base = rn32(x)
+    u = ulp32(base)
+    out = []
+    for k in range(-3, 4):
+        y = base + k * u
+        if y > 0 and abs(y - x) < ulp32(y):
+            out.append(y)
+    assert out
+    return out
+
+
+def rand32(rng, lo_exp, hi_exp):
+    e = rng.randint(lo_exp, hi_exp - 1)
+    m = rng.randint(2 ** (P - 1), 2 ** P - 1)
+    return Fraction(m) * Fraction(2) ** (e - P + 1)
+
+
+def hard_case(rng):
+    """n, d with n / d within ~2^-48 (relative) of a rounding midpoint: with
+    D odd, X = r0 / D mod 2^24 makes D X = N 2^24 + r0 for a small odd r0,
+    so n / d = X / 2^24 - r0 / (2^24 D) sits just beside the midpoint X / 2^24
+    (X odd, 25 bits) of two FP32 values."""
+    while True:
+        D = rng.randrange(2 ** 23 + 1, 2 ** 24, 2)
+        r0 = rng.choice((1, -1, 3, -3, 5, -5))
+        X = (r0 * pow(D, -1, 2 ** 24)) % 2 ** 24 + 2 ** 24
+        N, rem = divmod(D * X - r0, 2 ** 24)
+        if rem == 0 and 2 ** 23 <= N < 2 ** 24:
+            break
+    a = rng.randint(-60, 40)
+    b = rng.randint(-60 - a, 40 - a)
+    return Fraction(N) * Fraction(2) ** (a + b), Fraction(D) * Fraction(2) ** (b - 23)
+
+
+def cases(seed, count):
+    rng = random.Random(seed)
+    out = []
+    for _ in range(count):
+        if rng.random() < 0.5:
+            n, d = hard_case(rng)
+        else:
+            d = rand32(rng, -60, 60)
+            n = rand32(rng, -100, 60)
+        if rng.random() < 0.3:
+            n = -n
+        qe = n / d
+        if qe != 0 and not (Fraction(2) ** -125 <= abs(qe) < Fraction(2) ** 96):
+            continue
+        out.append((n, d))
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_short_division_is_the_ieee_quotient(seed):
+    for n, d in cases(seed, 1000):
+        want = rn32(n / d)
+        for y0 in recips(d):
+            assert seq(n, d, y0) == want, (n, d, y0)
+
+
+def test_zero_numerator_gives_plus_zero():
+    for d in (Fraction(1), Fraction(3, 7), Fraction(2) ** -100, Fraction(2) ** 100):
+        for y0 in recips(rn32(d)):
+            assert seq(Fraction(0), rn32(d), y0) == 0
+
+
+def test_the_cases_need_the_refined_reciprocal():
+    """With the raw reciprocal (no Newton step) some of the near-midpoint
+    quotients round the wrong way, so the cases above can tell a sound
+    sequence from a plausible shortcut."""
+    wrong = 0
+    for n, d in cases(7, 600):
+        want = rn32(n / d)
+        if any(seq(n, d, y0, newton=False) != want for y0 in recips(d)):
+            wrong += 1
+    assert wrong > 0
