@@ -35,8 +35,13 @@ namespace ausm {
 // LIM: the limiter as a compile-time constant (the fast path's van Albada,
 // the reference's own reconstruction), or -1 for the run-time choice.
 // STAGE: the RK stage as a compile-time constant (1..3), or -1 for a.stage.
-template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1>
-__global__ __launch_bounds__(256) void stage_kernel(const StageArgs a) {
+// WPE: the wavefronts per SIMD the register allocation must allow (the
+// exact-math van Albada stages of one and two rows fit 4 in 128 VGPRs
+// without spilling, against 3 at their unconstrained 140-170; every other
+// variant is left to the compiler).
+template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1, int WPE = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(
+    const StageArgs a) {
   const DevParams P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x;
@@ -90,9 +95,10 @@ int stage_blocks(int M, int N, int ty) {
 template <int SCHEME, bool FAST, int LIM, int STAGE>
 static hipError_t launch_ty_lim(const StageArgs& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
+  constexpr int W = (!FAST && LIM == LIM_VANALBADA && STAGE > 0) ? 4 : 1;
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1, LIM, STAGE, W>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2, LIM, STAGE, W>), dim3(nb), dim3(256), 0, s, a); break;
     case 4: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 4, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
     case 8: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 8, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
