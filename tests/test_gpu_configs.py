@@ -176,3 +176,53 @@ def test_headline_2000_steps_bitwise_and_fast_within_the_rounding_floor():
     assert ref_vs_64 > 1e-6  # the floor is real, not zero
     assert fast_vs_ref < 2 * ref_vs_64
     s.close()
+
+
+def test_naca0012_converged_plausibility():
+    """The plain-method configuration (NACA 0012, 256 x 128, 20-chord
+    circular far field, Mach 0.8, AoA 1.25 deg, SLAU2 + the reference's van
+    Albada) run to a steady state: local time step, compensated FP32 state,
+    fast math.  The limited solve cycles at a ~2e-2 residual drop with Cl
+    swinging by ~1e-3; frozen after 30000 steps (ausm_freeze_limiter) it
+    converges below a 1e-5 drop.  Its Cl and Cd sit in the limit cycle's
+    range and within 10 % and 15 % of the AGARD-AR-211 Euler values
+    (Cl 0.3632, Cd 0.0230, finer grids with the far field much further out);
+    the first-order solve, far more dissipative, loses over a tenth of the
+    lift and gains over 40 % in drag."""
+    s = ausm.Solver(M, N)
+    s.update_grid_boundaries(ausm.naca4_boundary(M, code=12, size=0.5, spacing="cosine"),
+                             ausm.circle_boundary(M, 20.0))
+    s.prepare_grid(iters=2000, method="sor", omega=1.5)
+    s.set_flow(0.8, 1.25)
+    s.set_cfl(0.5, 1.0, 1)
+    s.set_math("fast")
+    s.set_dt_mode("local")
+    s.set_precision("fp32c")
+    out = {}
+    for name, lim in (("muscl", VANALBADA), ("first", ausm.NOLIMITER)):
+        s.set_scheme(SLAU2, lim, 0.0, 0.75)
+        s.prepare_state(True)
+        r0 = s.residual_norm()
+        cls = []
+        for k in range(12):
+            s.step(2500)
+            if k >= 8:
+                cls.append(s.forces(1.0)["cl"])
+        if lim == VANALBADA:
+            s.freeze_limiter()
+        drop, steps = 1.0, 0
+        while drop > 1e-5 and steps < 60000:
+            drop = s.step(2500)["res_l2"] / r0
+            steps += 2500
+        f = s.forces(1.0)
+        out[name] = (drop, f["cl"], f["cd"], min(cls), max(cls))
+        print(f"[c1] {name}: drop {drop:.2e} after {30000 + steps} steps, Cl {f['cl']:.5f} Cd "
+              f"{f['cd']:.5f} (Cl over the last 10000 limited steps {min(cls):.5f} .. {max(cls):.5f})",
+              flush=True)
+        assert drop <= 1e-5
+    _, cl, cd, lo, hi = out["muscl"]
+    assert lo - 1e-3 <= cl <= hi + 1e-3
+    assert abs(cl - 0.3632) < 0.1 * 0.3632 and abs(cd - 0.0230) < 0.15 * 0.0230
+    _, cl1, cd1, _, _ = out["first"]
+    assert cl1 < 0.9 * cl and cd1 > 1.4 * cd
+    s.close()
