@@ -111,7 +111,7 @@ struct Slot {
 };
 
 struct BGraph {
-  int steps, ty, blocks, math, comp;
+  int steps, ty, blocks, math, comp, frz;
   hipGraph_t g;
   hipGraphExec_t e;
 };
@@ -174,6 +174,7 @@ struct ausm_batch {
   size_t seg_cap = 0;
   bool use_graphs = true;    // AUSM_NO_GRAPHS set: chunks are launched directly
   int max_items = 0;
+  int n_frozen = 0;  // running solves with the limiter frozen (kernel variant)
   int n_items = 0;
   bool dirty = false;
   int math = AUSM_MATH_FAST;
@@ -367,10 +368,10 @@ static float time_schedule(ausm_batch* b, int ty, int bpc) {
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   const int blocks = b->cus * bpc;
-  (void)launch_batch_stage(a4, b->math, b->comp, ty, blocks, b->stream);
+  (void)launch_batch_stage(a4, b->math, b->comp, b->n_frozen > 0, ty, blocks, b->stream);
   (void)hipEventRecord(e0, b->stream);
   for (const BatchArgs* a : {&a1, &a2, &a4})
-    (void)launch_batch_stage(*a, b->math, b->comp, ty, blocks, b->stream);
+    (void)launch_batch_stage(*a, b->math, b->comp, b->n_frozen > 0, ty, blocks, b->stream);
   (void)hipEventRecord(e1, b->stream);
   (void)hipEventSynchronize(e1);
   float ms = 1e30f;
@@ -459,7 +460,7 @@ static hipError_t enqueue_steps(ausm_batch* b, int nsteps) {
     for (int st = 1; st <= 3 && e == hipSuccess; ++st) {
       // the convergence check after the chunk reads the last step's norm
       BatchArgs a{b->d_solves, b->d_items, b->d_counts, st, st == 3, st == 1 && s == nsteps - 1};
-      e = launch_batch_stage(a, b->math, b->comp, b->ty, blocks, b->stream);
+      e = launch_batch_stage(a, b->math, b->comp, b->n_frozen > 0, b->ty, blocks, b->stream);
     }
     if (e != hipSuccess) return e;
   }
@@ -475,10 +476,10 @@ static int run_chunk(ausm_batch* b, int nsteps) {
   BGraph* g = nullptr;
   for (auto& x : b->graphs)
     if (x.steps == nsteps && x.ty == b->ty && x.blocks == blocks && x.math == b->math &&
-        x.comp == (int)b->comp)
+        x.comp == (int)b->comp && x.frz == (b->n_frozen > 0))
       g = &x;
   if (!g) {
-    BGraph n{nsteps, b->ty, blocks, b->math, (int)b->comp, nullptr, nullptr};
+    BGraph n{nsteps, b->ty, blocks, b->math, (int)b->comp, b->n_frozen > 0, nullptr, nullptr};
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return set_error(AUSM_ERR_HIP, "graph capture failed");
     const hipError_t e = enqueue_steps(b, nsteps);
@@ -524,6 +525,7 @@ static int retire(ausm_batch* b, Slot* s, int state) {
     s->cl = b->h_tmpd[2];
     s->cd = b->h_tmpd[3];
   }
+  if (s->frozen) --b->n_frozen;
   b->h_solves[s->index].active = 0;
   b->table[s->index] = nullptr;
   s->index = -1;
@@ -544,6 +546,7 @@ static int freeze_solve(ausm_batch* b, Slot* s) {
   if (launch_stage(a, s->d.scheme, b->math, 2, b->stream) != hipSuccess)
     return set_error(AUSM_ERR_HIP, "limiter record failed");
   s->frozen = true;
+  ++b->n_frozen;
   s->dev.freeze = 2;
   b->h_solves[s->index].freeze = 2;
   if (hipMemcpyAsync(b->d_solves + s->index, &b->h_solves[s->index], sizeof(SolveDev),
@@ -555,6 +558,7 @@ static int freeze_solve(ausm_batch* b, Slot* s) {
 
 static void free_slot(ausm_batch* b, Slot* s) {
   if (s->index >= 0) {
+    if (s->frozen) --b->n_frozen;
     b->h_solves[s->index].active = 0;
     b->table[s->index] = nullptr;
     b->dirty = true;

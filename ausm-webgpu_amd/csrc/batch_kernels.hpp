@@ -95,8 +95,9 @@ __host__ __device__ inline StageArgs batch_args(const SolveDev& S, int stage, in
   return a;
 }
 
-// comp: compensated-state stages (every solve of the batch)
-hipError_t launch_batch_stage(const BatchArgs& b, int fast, int comp, int ty, int blocks,
+// comp: compensated-state stages (every solve of the batch); frz: some
+// solve of the batch has its limiter frozen
+hipError_t launch_batch_stage(const BatchArgs& b, int fast, int comp, int frz, int ty, int blocks,
                               hipStream_t s);
 hipError_t launch_batch_dt(const SolveDev* solves, const int* n_solves, int max_solves,
                            hipStream_t s);

@@ -38,8 +38,9 @@ namespace ausm {
 // WPE: the wavefronts per SIMD the register allocation must allow (the
 // exact-math van Albada stages of one and two rows fit 4 in 128 VGPRs
 // without spilling, against 3 at their unconstrained 140-170; every other
-// variant is left to the compiler).
-template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1, int WPE = 1>
+// variant is left to the compiler).  FRZ: limiter freezing compiled in
+// (launched for frozen solves and recording passes only).
+template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1, int WPE = 1, bool FRZ = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(
     const StageArgs a) {
   const DevParams P = *a.par;
@@ -49,14 +50,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const int stage = STAGE >= 0 ? STAGE : a.stage;
   const DtAhead<float> ahead = stage_dt_load(a, stage);
   __shared__ FaceLds<float> s_face;
-  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, false, LIM, STAGE>(
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, false, LIM, STAGE, FRZ>(
       P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
   if (stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
 // All-FP64 variant (exact arithmetic): the same tile walk over double
 // state and vertex planes, for the FP64 comparison solve.
-template <int SCHEME, int TY, int LIM = -1, int STAGE = -1>
+template <int SCHEME, int TY, int LIM = -1, int STAGE = -1, bool FRZ = false>
 __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
   const DevParamsT<double> P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -65,13 +66,13 @@ __global__ __launch_bounds__(256) void stage_kernel64(const StageArgs64 a) {
   const int stage = STAGE >= 0 ? STAGE : a.stage;
   const DtAhead<double> ahead = stage_dt_load(a, stage);
   __shared__ FaceLds<double> s_face;
-  const double ws_acc = stage_tile<SCHEME, false, TY, double, false, LIM, STAGE>(
+  const double ws_acc = stage_tile<SCHEME, false, TY, double, false, LIM, STAGE, FRZ>(
       P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
   if (stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
 
 // Compensated-state FP32 variant (the COMP stages of stage_tile).
-template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1>
+template <int SCHEME, bool FAST, int TY, int LIM = -1, int STAGE = -1, bool FRZ = false>
 __global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
   const DevParams P = *a.par;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(256) void stage_kernel_comp(const StageArgs a) {
   const int stage = STAGE >= 0 ? STAGE : a.stage;
   const DtAhead<float> ahead = stage_dt_load(a, stage);
   __shared__ FaceLds<float> s_face;
-  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true, LIM, STAGE>(
+  const float ws_acc = stage_tile<SCHEME, FAST, TY, float, true, LIM, STAGE, FRZ>(
       P, a, tx, tyb, s_face, [&] { return stage_dt_finish(P, a, blockIdx.x == 0, ahead, stage); });
   if (stage == 3 && a.refresh) block_ws_partial(ws_acc, a.ws_out + blockIdx.x);
 }
@@ -92,15 +93,15 @@ int stage_blocks(int M, int N, int ty) {
   return stage_tiles_x(M) * ((N + rows_per_block - 1) / rows_per_block);
 }
 
-template <int SCHEME, bool FAST, int LIM, int STAGE>
+template <int SCHEME, bool FAST, int LIM, int STAGE, bool FRZ = false>
 static hipError_t launch_ty_lim(const StageArgs& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   constexpr int W = (!FAST && LIM == LIM_VANALBADA && STAGE > 0) ? 4 : 1;
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1, LIM, STAGE, W>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2, LIM, STAGE, W>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 4, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 8, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 1, LIM, STAGE, W, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 2, LIM, STAGE, W, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 4, LIM, STAGE, 1, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((stage_kernel<SCHEME, FAST, 8, LIM, STAGE, 1, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -112,7 +113,8 @@ static hipError_t launch_ty_lim(const StageArgs& a, int ty, hipStream_t s) {
 // freezing) the general kernel
 template <int SCHEME, bool FAST>
 static hipError_t launch_ty(const StageArgs& a, int ty, hipStream_t s) {
-  if (a.limiter == LIM_VANALBADA && a.freeze == 0) {
+  if (a.freeze) return launch_ty_lim<SCHEME, FAST, -1, -1, true>(a, ty, s);
+  if (a.limiter == LIM_VANALBADA) {
     switch (a.stage) {
       case 1: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
       case 2: return launch_ty_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
@@ -141,13 +143,13 @@ hipError_t launch_stage(const StageArgs& a, int scheme, int fast, int ty, hipStr
   return hipErrorInvalidValue;
 }
 
-template <int SCHEME, bool FAST, int LIM, int STAGE>
+template <int SCHEME, bool FAST, int LIM, int STAGE, bool FRZ = false>
 static hipError_t launch_ty_comp_lim(const StageArgs& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 1, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 2, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 4, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 1, LIM, STAGE, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 2, LIM, STAGE, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel_comp<SCHEME, FAST, 4, LIM, STAGE, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -156,8 +158,9 @@ static hipError_t launch_ty_comp_lim(const StageArgs& a, int ty, hipStream_t s) 
 // fast van Albada stages 1-3 specialised as in launch_ty
 template <int SCHEME, bool FAST>
 static hipError_t launch_ty_comp(const StageArgs& a, int ty, hipStream_t s) {
+  if (a.freeze) return launch_ty_comp_lim<SCHEME, FAST, -1, -1, true>(a, ty, s);
   if constexpr (FAST) {
-    if (a.limiter == LIM_VANALBADA && a.freeze == 0) {
+    if (a.limiter == LIM_VANALBADA) {
       switch (a.stage) {
         case 1: return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA, 1>(a, ty, s);
         case 2: return launch_ty_comp_lim<SCHEME, FAST, LIM_VANALBADA, 2>(a, ty, s);
@@ -182,13 +185,13 @@ hipError_t launch_stage_comp(const StageArgs& a, int scheme, int fast, int ty, h
   return hipErrorInvalidValue;
 }
 
-template <int SCHEME, int LIM, int STAGE>
+template <int SCHEME, int LIM, int STAGE, bool FRZ = false>
 static hipError_t launch_ty64_lim(const StageArgs64& a, int ty, hipStream_t s) {
   const int nb = stage_blocks(a.M, a.N, ty);
   switch (ty) {
-    case 1: hipLaunchKernelGGL((stage_kernel64<SCHEME, 1, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel64<SCHEME, 2, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel64<SCHEME, 4, LIM, STAGE>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel64<SCHEME, 1, LIM, STAGE, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel64<SCHEME, 2, LIM, STAGE, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel64<SCHEME, 4, LIM, STAGE, FRZ>), dim3(nb), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -197,7 +200,8 @@ static hipError_t launch_ty64_lim(const StageArgs64& a, int ty, hipStream_t s) {
 // van Albada stages 1-3 specialised as in launch_ty
 template <int SCHEME>
 static hipError_t launch_ty64(const StageArgs64& a, int ty, hipStream_t s) {
-  if (a.limiter == LIM_VANALBADA && a.freeze == 0) {
+  if (a.freeze) return launch_ty64_lim<SCHEME, -1, -1, true>(a, ty, s);
+  if (a.limiter == LIM_VANALBADA) {
     switch (a.stage) {
       case 1: return launch_ty64_lim<SCHEME, LIM_VANALBADA, 1>(a, ty, s);
       case 2: return launch_ty64_lim<SCHEME, LIM_VANALBADA, 2>(a, ty, s);

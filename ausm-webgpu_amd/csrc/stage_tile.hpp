@@ -186,9 +186,12 @@ using FaceLds = R[kWavesPerBlock][5][kWave];
 
 // dt_arg is the stage's dt, or a callable that returns it; a callable is
 // evaluated once, after the workgroup barrier, so that a reduction behind it
-// can overlap the tile's loads and first face.
+// can overlap the tile's loads and first face.  FRZ: the limiter-freezing
+// record and reconstruction (a.freeze) are compiled in; only code that
+// steps frozen solves instantiates it (with the run-time limiter), so the
+// other variants carry neither its code nor its registers.
 template <int SCHEME, bool FAST, int TY, typename R = float, bool COMP = false, int LIM = -1,
-          int STAGE = -1, typename DtF = R>
+          int STAGE = -1, bool FRZ = false, typename DtF = R>
 __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT<R>& a, int tx,
                                         int tyb, FaceLds<R>& s_face, DtF dt_arg) {
   static_assert(!FAST || sizeof(R) == 4, "the fast arithmetic is FP32 only");
@@ -343,12 +346,12 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
 #pragma unroll
     for (int k = 1; k < NR; ++k)
       if (k > 1 || rb == 0) dd[k] = pdiff(w[k], w[k - 1]);
-    // limiter freezing (kernels with the run-time limiter only): a frozen
-    // face loads its 12 recorded values where it is evaluated, and a
-    // recording pass stores them at the lanes that own the column.  Planes
-    // 0-11: vertical faces, row jf = face between rows jf-1 and jf; planes
-    // 12-23: horizontal faces, row r = left face of the cell.
-    constexpr bool FRZ = LIM < 0;
+    // limiter freezing: a frozen face loads its 12 recorded values where it
+    // is evaluated, and a recording pass stores them at the lanes that own
+    // the column.  Planes 0-11: vertical faces, row jf = face between rows
+    // jf-1 and jf; planes 12-23: horizontal faces, row r = left face of the
+    // cell.
+    static_assert(!FRZ || LIM < 0, "freezing runs with the run-time limiter");
     const int frz = FRZ ? a.freeze : 0;
     const size_t FPl = psi_plane(N, Pp);
     auto get_psi = [&](size_t plane0, int row, R* ps) {
