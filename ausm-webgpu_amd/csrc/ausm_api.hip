@@ -845,6 +845,94 @@ int ausm_get_state64(ausm_ctx* c, double* q) {
   return AUSM_OK;
 }
 
+namespace ausm {
+
+// The graph of `len` steps (len a multiple of the refresh cadence K) run from
+// a state at a refresh boundary whose last refresh left n_ws_in per-workgroup
+// maxima: found in the context's cache, or captured and instantiated (nothing
+// runs).  Within the chunk step s is fresh iff s % K == 0.
+static int graph_for(ausm_ctx* c, int ty, int len, int n_ws_in, GraphEntry** out) {
+  for (auto& g : c->graphs)
+    if (g.ty == ty && g.steps == len && g.n_ws_in == n_ws_in) {
+      *out = &g;
+      return AUSM_OK;
+    }
+  if (c->graphs.size() >= kMaxGraphs) {
+    (void)hipStreamSynchronize(c->stream);
+    // oldest first: the full-length chunks are captured first and
+    // recaptured at once if they are ever needed again
+    hipGraphExecDestroy(c->graphs.front().exec);
+    hipGraphDestroy(c->graphs.front().graph);
+    c->graphs.erase(c->graphs.begin());
+  }
+  const int K = c->refresh;
+  GraphEntry g{};
+  g.ty = ty;
+  g.steps = len;
+  g.n_ws_in = n_ws_in;
+  const int save_valid = c->n_ws_valid;
+  c->n_ws_valid = n_ws_in;
+  CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < len && e == hipSuccess; ++s) {
+    const bool fresh = (s % K) == 0;
+    const bool refresh = ((s + 1) % K) == 0;
+    e = enqueue_step(c, ty, fresh, refresh, s == len - 1);
+    if (refresh) c->n_ws_valid = stage_blocks(c->M, c->N, ty);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+  c->n_ws_valid = save_valid;
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return e != hipSuccess ? hip_fail(e, "graph capture (launch)") : hip_fail(e2, "graph capture");
+  }
+  g.graph = graph;
+  CHECK_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0), "graph instantiate");
+  c->graphs.push_back(g);
+  *out = &c->graphs.back();
+  return AUSM_OK;
+}
+
+// graph chunk: a multiple of the dt-refresh cadence, about 32 steps; a
+// shorter remainder of whole refresh periods is a graph of its own, so only
+// the steps of a partial period run as single launches
+static int chunk_steps(int K) { return K * std::max(1, 32 / K); }
+
+}  // namespace ausm
+
+int ausm_prepare_steps(ausm_ctx* c, int nsteps) {
+  if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
+  if (nsteps < 0) return set_error(AUSM_ERR_ARG, "negative step count");
+  if (!c->has_state) return set_error(AUSM_ERR_ORDER, "initialise the state before stepping");
+  if (!c->use_graphs) return AUSM_OK;
+  hipSetDevice(c->device);
+  const int ty = current_rows(c);
+  const int K = c->refresh;
+  const int G = chunk_steps(K);
+  // the chunk sequence ausm_step would run from the current counters
+  int since = c->since_refresh, valid = c->n_ws_valid;
+  bool ws_fresh = c->ws_fresh;
+  int left = nsteps;
+  while (left > 0) {
+    if (since == 0 && ws_fresh && left >= K) {
+      const int len = left >= G ? G : left / K * K;
+      GraphEntry* ge = nullptr;
+      const int r = graph_for(c, ty, len, valid, &ge);
+      if (r != AUSM_OK) return r;
+      left -= len;
+      valid = stage_blocks(c->M, c->N, ty);
+    } else {
+      const bool refresh = since + 1 >= K;
+      ws_fresh = refresh;
+      since = refresh ? 0 : since + 1;
+      if (refresh) valid = stage_blocks(c->M, c->N, ty);
+      left -= 1;
+    }
+  }
+  return AUSM_OK;
+}
+
 int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
   if (!c) return set_error(AUSM_ERR_ARG, "NULL context");
   if (nsteps < 0) return set_error(AUSM_ERR_ARG, "negative step count");
@@ -852,53 +940,15 @@ int ausm_step(ausm_ctx* c, int nsteps, ausm_step_info* info) {
   hipSetDevice(c->device);
   const int ty = current_rows(c);
   const int K = c->refresh;
-  // graph chunk: a multiple of the dt-refresh cadence, about 32 steps; a
-  // shorter remainder of whole refresh periods is a graph of its own, so
-  // only the steps of a partial period run as single launches
-  const int G = K * std::max(1, 32 / K);
+  const int G = chunk_steps(K);
   int left = nsteps;
   while (left > 0) {
     const bool aligned = c->since_refresh == 0 && c->ws_fresh;
     if (aligned && left >= K && c->use_graphs) {
       const int len = left >= G ? G : left / K * K;
       GraphEntry* ge = nullptr;
-      for (auto& g : c->graphs)
-        if (g.ty == ty && g.steps == len && g.n_ws_in == c->n_ws_valid) ge = &g;
-      if (!ge) {
-        if (c->graphs.size() >= kMaxGraphs) {
-          (void)hipStreamSynchronize(c->stream);
-          // oldest first: the full-length chunks are captured first and
-          // recaptured at once if they are ever needed again
-          hipGraphExecDestroy(c->graphs.front().exec);
-          hipGraphDestroy(c->graphs.front().graph);
-          c->graphs.erase(c->graphs.begin());
-        }
-        // capture len steps; within the chunk step s is fresh iff s % K == 0
-        GraphEntry g{};
-        g.ty = ty;
-        g.steps = len;
-        g.n_ws_in = c->n_ws_valid;
-        CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-        const int save_valid = c->n_ws_valid;
-        hipError_t e = hipSuccess;
-        for (int s = 0; s < len && e == hipSuccess; ++s) {
-          const bool fresh = (s % K) == 0;
-          const bool refresh = ((s + 1) % K) == 0;
-          e = enqueue_step(c, ty, fresh, refresh, s == len - 1);
-          if (refresh) c->n_ws_valid = stage_blocks(c->M, c->N, ty);
-        }
-        hipGraph_t graph = nullptr;
-        hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
-        c->n_ws_valid = save_valid;
-        if (e != hipSuccess || e2 != hipSuccess) {
-          if (graph) hipGraphDestroy(graph);
-          return e != hipSuccess ? hip_fail(e, "graph capture (launch)") : hip_fail(e2, "graph capture");
-        }
-        g.graph = graph;
-        CHECK_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0), "graph instantiate");
-        c->graphs.push_back(g);
-        ge = &c->graphs.back();
-      }
+      const int r = graph_for(c, ty, len, c->n_ws_valid, &ge);
+      if (r != AUSM_OK) return r;
       CHECK_HIP(hipGraphLaunch(ge->exec, c->stream), "graph launch");
       c->steps += len;
       left -= len;

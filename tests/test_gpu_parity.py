@@ -124,6 +124,28 @@ def test_steps_exact_bitwise(scheme, mach, aoa):
     s.close()
 
 
+@pytest.mark.parametrize("refresh", [1, 7])
+def test_prepared_graphs_step_bitwise(refresh):
+    """ausm_prepare_steps runs nothing: the state is untouched, and the steps
+    that then launch the prepared graphs (a partial dt period, whole periods
+    and a remainder) equal the oracle's."""
+    M, N = 70, 24
+    vtx, _, _ = make_grid(M, N, code=4412, iters=100)
+    s, ref, par = new_pair(vtx, 0.6, 3.0, SLAU2, VANALBADA, refresh=refresh)
+    st = perturbed_state(par, M, N, seed=9)
+    s.set_state(st)
+    ref.set_state(st)
+    before = s.state()
+    for chunk in (3, 20, 45):
+        s.prepare_steps(chunk)
+        assert np.array_equal(s.state(), before)
+        s.step(chunk, info=False)
+        ref.step(chunk)
+        before = s.state()
+        assert np.array_equal(before[1:N + 1], ref.state[1:N + 1]), chunk
+    s.close()
+
+
 def test_dt_refresh_cadence_bitwise():
     """steps_per_dt_refresh = 7 (the reference refreshes dt once per frame)."""
     M, N = 70, 24
