@@ -564,4 +564,27 @@ template hipError_t launch_convert_planes<float, double>(const float*, double*, 
 template hipError_t launch_convert_planes<double, float>(const double*, float*, int, int, int, int,
                                                          hipStream_t);
 
+// --------------------------------------------------------------------------
+// self-check of the exact arithmetic's refined reciprocal
+// --------------------------------------------------------------------------
+// One thread per FP32 divisor d = 2^(e-127) * 1.m, e = 1 + blockIdx.y
+// (d in [2^-126, 2^126)), m = blockIdx.x * 256 + threadIdx.x: recip_rn(d)
+// against the IEEE quotient 1/d; each wavefront with a mismatch adds its
+// count to *bad (nothing is written when every divisor agrees).
+constexpr int kDivCheckExponents = 252;
+__global__ __launch_bounds__(256) void division_check_kernel(unsigned long long* bad) {
+  const unsigned m = blockIdx.x * 256u + threadIdx.x;
+  const unsigned e = 1u + blockIdx.y;
+  const float d = __uint_as_float((e << 23) | m);
+  const float y = recip_rn(d);
+  const float ref = 1.0f / d;  // the compiler's IEEE division
+  const unsigned long long w = __ballot(__float_as_uint(y) != __float_as_uint(ref));
+  if ((threadIdx.x & 63) == 0 && w != 0ull) atomicAdd(bad, (unsigned long long)__popcll(w));
+}
+hipError_t launch_division_check(unsigned long long* bad, hipStream_t s) {
+  hipLaunchKernelGGL(division_check_kernel, dim3((1u << 23) / 256u, kDivCheckExponents), dim3(256), 0,
+                     s, bad);
+  return hipGetLastError();
+}
+
 }  // namespace ausm

@@ -49,4 +49,8 @@ hipError_t launch_comp_to_f64(const float* hi, const float* lo, double* q, int M
 hipError_t launch_f64_to_comp(const double* q, float* hi, float* lo, int M, int N, int P, int PS,
                               hipStream_t s);
 
+// recip_rn(d) against the IEEE 1/d for every FP32 divisor in [2^-126, 2^126);
+// adds the number of mismatches to *bad (which the caller clears)
+hipError_t launch_division_check(unsigned long long* bad, hipStream_t s);
+
 }  // namespace ausm

@@ -61,21 +61,25 @@ using Flux = FluxT<float>;
 // a Newton step on the reciprocal, a quotient with two FMA-residual
 // corrections, v_div_fmas and v_div_fixup: 11 VALU instructions, of which
 // the two scalings and the fixup only act on operands near the ends of the
-// exponent range (denormal or huge divisor, numerator below 2^-103, quotient
+// exponent range (denormal or huge divisor, numerator below 2^-100, quotient
 // near overflow or underflow) and on zeros, infinities and NaNs.  div_rn is
-// the same sequence without them (8 instructions):
-//   y = rcp(d);  y += y (1 - d y);  q = n y;  q += y (n - d q);  q += y (n - d q)
-// and returns the IEEE quotient, bit for bit, whenever d > 0 is normal with
-// d < 2^126, n = +0 or 2^-100 <= |n|, and |n / d| < 2^96: the residuals
-// n - d q are then exact, and two corrections of a quotient within an ulp,
-// with a reciprocal within an ulp, round correctly (Markstein).  div_by
-// takes the refined reciprocal recip_rn(d), so that quotients with one
-// divisor share it.  Every call site states why its operands are in range;
-// elsewhere the exact path keeps the compiler's division (quotients whose
-// numerators can be arbitrarily small, such as momenta and velocities at
-// the front of a disturbance spreading into a uniform flow).  A zero,
-// infinite or NaN divisor or an infinite numerator gives NaN where IEEE
-// may give an infinity: a state that reaches them is non-finite either
+// 6 instructions:
+//   y = rcp(d);  y += y (1 - d y);  q = n y;  q += y (n - d q)
+// The refined reciprocal y = recip_rn(d) is the correctly rounded 1/d for
+// every normal d < 2^126 on gfx950: v_rcp_f32 followed by that Newton step
+// was checked for every FP32 divisor in the range (ausm_check_division runs
+// the check on the device; contexts and batches refuse a device that fails
+// it).  With y = RN(1/d), a quotient q = RN(n y) within an ulp and an exact
+// residual n - d q, one correction rounds correctly (Markstein's theorem).
+// The residual is exact, and the result the IEEE quotient bit for bit,
+// whenever d > 0 is normal with d < 2^126, n = +0 or 2^-100 <= |n|, and
+// |n / d| < 2^96.  div_by takes the refined reciprocal, so that quotients
+// with one divisor share it.  Every call site states why its operands are
+// in range; elsewhere the exact path keeps the compiler's division
+// (quotients whose numerators can be arbitrarily small, such as momenta and
+// velocities at the front of a disturbance spreading into a uniform flow).
+// A zero, infinite or NaN divisor or an infinite numerator gives NaN where
+// IEEE may give an infinity: a state that reaches them is non-finite either
 // way.  R = double (the FP64 solve) always divides with the compiler's
 // sequence.
 // ---------------------------------------------------------------------------
@@ -91,8 +95,7 @@ __device__ __forceinline__ R recip_rn(R d) {
 template <typename R>
 __device__ __forceinline__ R div_by(R n, R d, R y) {
   if constexpr (sizeof(R) == 4) {
-    float q = n * y;
-    q = fmaf(fmaf(-d, q, n), y, q);
+    const float q = n * y;
     return fmaf(fmaf(-d, q, n), y, q);
   } else {
     (void)y;

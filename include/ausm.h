@@ -267,6 +267,15 @@ int ausm_set_state64(ausm_ctx* ctx, const double* q);
 int ausm_get_state64(ausm_ctx* ctx, double* q);
 int ausm_eval_residual64(ausm_ctx* ctx, double* res);
 
+/* Self-check of the device arithmetic exact math relies on, no counterpart
+ * in the reference: the exact path divides with a refined hardware
+ * reciprocal that must be the correctly rounded 1/d.  Runs the check for
+ * every FP32 divisor in [2^-126, 2^126) on `device` and returns the number
+ * of divisors whose refined reciprocal differs (0 on gfx950).  Context and
+ * batch creation run it once per device and refuse a device that fails;
+ * callers of ausm_step_noalloc can run it once at set-up. */
+int ausm_check_division(int device, long long* mismatches);
+
 /* Timing hook, no counterpart in the reference (it times whole passes with
  * timestamp queries, main.js:597-602): launches each of the three stage
  * kernels `reps` times back to back on the context's stream between one

@@ -3,20 +3,23 @@ device_math.hpp), checked in exact rational arithmetic.
 
 The device sequence is
     y0 = v_rcp_f32(d)                  (within 1 ulp of 1/d)
-    y  = fma(fma(-d, y0, 1), y0, y0)
+    y  = fma(fma(-d, y0, 1), y0, y0)   (= RN(1/d): the device premise)
     q0 = n * y
-    q1 = fma(fma(-d, q0, n), y, q0)
-    q  = fma(fma(-d, q1, n), y, q1)
+    q  = fma(fma(-d, q0, n), y, q0)
 with every operation rounded once to FP32 (round to nearest, ties to even).
 It claims the IEEE quotient RN(n / d) bit for bit for d > 0 normal with
-d < 2^126, n = +0 or 2^-100 <= |n|, |n / d| < 2^96.  Here each operation is
-evaluated exactly and rounded to FP32 by hand, for every reciprocal the
-hardware may return (each FP32 value within 1 ulp of 1/d), over random
-operands spanning the claimed range and over quotients placed next to a
-rounding midpoint (constructed to within ~2^-48 of it: the cases on which
-the sequence without its Newton step rounds the wrong way, checked below
-too, so that the test can fail).  The end-to-end bitwise parity tests
-of the exact path cover the operands the solver actually divides.
+d < 2^126, n = +0 or 2^-100 <= |n|, |n / d| < 2^96, given the premise that
+the refined reciprocal y is the correctly rounded 1/d.  The premise is a
+property of the hardware reciprocal: ausm_check_division verifies it on
+the device for every FP32 divisor (tests/test_gpu_parity.py), and context
+and batch creation refuse a device that fails it.  Here, with y = RN(1/d),
+each operation is evaluated exactly and rounded to FP32 by hand, over
+random operands spanning the claimed range and over quotients placed next
+to a rounding midpoint (constructed to within ~2^-48 of it: the cases on
+which the same sequence with a reciprocal one ulp off rounds the wrong
+way, checked below too, so that the test can fail).  The end-to-end
+bitwise parity tests of the exact path cover the operands the solver
+actually divides.
 """
 import math
 import random
@@ -64,26 +67,22 @@ def fma(a, b, c):
     return rn32(a * b + c)
 
 
-def seq(n, d, y0, newton=True):
-    y = fma(fma(-d, y0, 1), y0, y0) if newton else y0
+def seq(n, d, y):
+    """the device quotient from the reciprocal y of d"""
     q = rn32(n * y)
-    for _ in range(2):
-        q = fma(fma(-d, q, n), y, q)
-    return q
+    return fma(fma(-d, q, n), y, q)
 
 
-def recips(d):
-    """Every FP32 value within one ulp of 1/d."""
-    x = 1 / d
-    base = rn32(x)
-    u = ulp32(base)
-    out = []
-    for k in range(-3, 4):
-        y = base + k * u
-        if y > 0 and abs(y - x) < ulp32(y):
-            out.append(y)
-    assert out
-    return out
+def recip_rn(d):
+    """RN(1/d), what the refined hardware reciprocal returns"""
+    return rn32(1 / d)
+
+
+def off_by_one_ulp(d):
+    """the two FP32 neighbours of RN(1/d)"""
+    y = recip_rn(d)
+    u = ulp32(y)
+    return [y - u, y + u]
 
 
 def rand32(rng, lo_exp, hi_exp):
@@ -129,25 +128,22 @@ def cases(seed, count):
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_short_division_is_the_ieee_quotient(seed):
-    for n, d in cases(seed, 1000):
-        want = rn32(n / d)
-        for y0 in recips(d):
-            assert seq(n, d, y0) == want, (n, d, y0)
+    for n, d in cases(seed, 1500):
+        assert seq(n, d, recip_rn(d)) == rn32(n / d), (n, d)
 
 
 def test_zero_numerator_gives_plus_zero():
     for d in (Fraction(1), Fraction(3, 7), Fraction(2) ** -100, Fraction(2) ** 100):
-        for y0 in recips(rn32(d)):
-            assert seq(Fraction(0), rn32(d), y0) == 0
+        assert seq(Fraction(0), rn32(d), recip_rn(rn32(d))) == 0
 
 
-def test_the_cases_need_the_refined_reciprocal():
-    """With the raw reciprocal (no Newton step) some of the near-midpoint
-    quotients round the wrong way, so the cases above can tell a sound
-    sequence from a plausible shortcut."""
+def test_the_cases_need_the_correctly_rounded_reciprocal():
+    """With a reciprocal one ulp off RN(1/d) some of the near-midpoint
+    quotients round the wrong way after one correction, so the cases above
+    can tell the premise from a plausible shortcut."""
     wrong = 0
     for n, d in cases(7, 600):
         want = rn32(n / d)
-        if any(seq(n, d, y0, newton=False) != want for y0 in recips(d)):
+        if any(seq(n, d, y) != want for y in off_by_one_ulp(d)):
             wrong += 1
     assert wrong > 0

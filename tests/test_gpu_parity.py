@@ -93,6 +93,16 @@ def max_rel(a, b):
 
 
 # --------------------------------------------------------------------------
+def test_refined_reciprocal_is_correctly_rounded():
+    """The exact arithmetic's premise on this device: v_rcp_f32 plus one
+    Newton step is the IEEE 1/d for every FP32 divisor in [2^-126, 2^126)."""
+    import ctypes
+
+    bad = ctypes.c_longlong(-1)
+    ausm.lib.check(ausm.lib.load().ausm_check_division(0, ctypes.byref(bad)))
+    assert bad.value == 0
+
+
 @pytest.mark.parametrize("scheme", SCHEMES)
 @pytest.mark.parametrize("limiter", LIMITERS)
 def test_residual_exact_bitwise(scheme, limiter):
