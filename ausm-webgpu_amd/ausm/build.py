@@ -20,8 +20,14 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libausm.so")
 BUILD_DIR = os.path.join(PKG, "_build")
 
-SOURCES = ["stage_kernels.hip", "batch_kernels.hip", "aux_kernels.hip", "ausm_api.hip",
-           "batch.hip", "noalloc.hip", "geometry.cpp"]
+# the stage kernels are compiled per scheme and the batched kernels per
+# arithmetic, so that the longest translation units build in parallel
+STAGE_SOURCES = ["stage_kernels_slau2.hip", "stage_kernels_slau.hip", "stage_kernels_ausmup.hip"]
+BATCH_SOURCES = ["batch_stage_fast_plain.hip", "batch_stage_fast_comp.hip",
+                 "batch_stage_exact_plain.hip", "batch_stage_exact_comp.hip"]
+SOURCES = STAGE_SOURCES + BATCH_SOURCES + ["stage_kernels.hip", "batch_kernels.hip",
+                                           "aux_kernels.hip", "ausm_api.hip", "batch.hip",
+                                           "noalloc.hip", "geometry.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AUSM_ARCH", "gfx950")
@@ -33,7 +39,7 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=
 # per-file additions: the single-grid stage kernels are scheduled for
 # instruction-level parallelism (1-3 % faster across sizes, 7 % for exact
 # math); the persistent batched kernel is not (3 % slower with it)
-EXTRA_FLAGS = {"stage_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+EXTRA_FLAGS = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in STAGE_SOURCES}
 
 
 def _compile(src):

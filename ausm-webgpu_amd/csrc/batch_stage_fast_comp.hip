@@ -1,0 +1,10 @@
+// Persistent batched stage kernels, fast math, compensated FP32 state.
+#include "batch_stage.hpp"
+
+namespace ausm {
+
+hipError_t launch_bt_fast_comp(const BatchArgs& b, int frz, int ty, int blocks, hipStream_t s) {
+  return frz ? launch_bt<true, true, true>(b, ty, blocks, s) : launch_bt<true, true, false>(b, ty, blocks, s);
+}
+
+}  // namespace ausm

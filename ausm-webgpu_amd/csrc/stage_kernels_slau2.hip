@@ -1,0 +1,19 @@
+// The fused RK-stage kernels of one scheme (SCHEME_SLAU2): FP32 exact and fast
+// math, compensated FP32 and FP64 (kernel templates in stage_kernels.hpp).
+#include "stage_kernels.hpp"
+
+namespace ausm {
+
+hipError_t launch_stage_slau2(const StageArgs& a, int fast, int ty, hipStream_t s) {
+  return fast ? launch_ty<SCHEME_SLAU2, true>(a, ty, s) : launch_ty<SCHEME_SLAU2, false>(a, ty, s);
+}
+
+hipError_t launch_stage_comp_slau2(const StageArgs& a, int fast, int ty, hipStream_t s) {
+  return fast ? launch_ty_comp<SCHEME_SLAU2, true>(a, ty, s) : launch_ty_comp<SCHEME_SLAU2, false>(a, ty, s);
+}
+
+hipError_t launch_stage64_slau2(const StageArgs64& a, int ty, hipStream_t s) {
+  return launch_ty64<SCHEME_SLAU2>(a, ty, s);
+}
+
+}  // namespace ausm
