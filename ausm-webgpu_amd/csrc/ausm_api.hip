@@ -927,6 +927,13 @@ static int graph_for(ausm_ctx* c, int ty, int len, int n_ws_in, GraphEntry** out
   }
   g.graph = graph;
   CHECK_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0), "graph instantiate");
+  // upload the executable graph to the device now, not at its first launch
+  const hipError_t eu = hipGraphUpload(g.exec, c->stream);
+  if (eu != hipSuccess) {
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(graph);
+    return hip_fail(eu, "graph upload");
+  }
   c->graphs.push_back(g);
   *out = &c->graphs.back();
   return AUSM_OK;
