@@ -113,7 +113,7 @@ SIGNATURES = {
     "ausm_get_state": (_i, [_vp, _vp]),
     "ausm_step": (_i, [_vp, _i, ctypes.POINTER(StepInfo)]),
     "ausm_prepare_steps": (_i, [_vp, _i]),
-    "ausm_check_division": (_i, [_i, ctypes.POINTER(ctypes.c_longlong)]),
+    "ausm_check_arithmetic": (_i, [_i, ctypes.POINTER(ctypes.c_longlong)]),
     "ausm_eval_residual": (_i, [_vp, _vp]),
     "ausm_wave_speeds": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
     "ausm_forces": (_i, [_vp, _d, _vp, _vp]),

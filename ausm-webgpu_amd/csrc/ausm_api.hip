@@ -44,38 +44,44 @@ int hip_fail(hipError_t e, const char* what) {
 size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // The exact arithmetic divides with a refined hardware reciprocal that must
-// be the correctly rounded 1/d (recip_rn, device_math.hpp); the check runs
-// over every FP32 divisor in [2^-126, 2^126) on the current device.
-static int division_check(long long* mismatches) {
+// be the correctly rounded 1/d (recip_rn, device_math.hpp) and takes square
+// roots with a short form of the IEEE sequence (sqrt_rn); the check runs
+// both over their whole ranges on the current device.
+static int arithmetic_check(long long* mismatches) {
   unsigned long long* bad = nullptr;
   hipError_t e = hipMalloc(&bad, sizeof(unsigned long long));
-  if (e != hipSuccess) return set_error(AUSM_ERR_NOMEM, "division check: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return set_error(AUSM_ERR_NOMEM, "arithmetic check: %s", hipGetErrorString(e));
   unsigned long long h = 0;
   e = hipMemset(bad, 0, sizeof(h));
-  if (e == hipSuccess) e = launch_division_check(bad, nullptr);
+  if (e == hipSuccess) e = launch_arithmetic_check(bad, nullptr);
   if (e == hipSuccess) e = hipMemcpy(&h, bad, sizeof(h), hipMemcpyDeviceToHost);
   (void)hipFree(bad);
-  if (e != hipSuccess) return hip_fail(e, "division check");
+  if (e != hipSuccess) return hip_fail(e, "arithmetic check");
   *mismatches = (long long)h;
   return AUSM_OK;
 }
 
-// once per device and process: 0 not run, 1 passed, 2 failed
-static std::atomic<int> g_div_state[64];
+// once per device and process: whether the check has run, and its mismatch count
+static std::atomic<long long> g_arith_bad[64] = {};
+static std::atomic<bool> g_arith_run[64] = {};
 
-int device_division_ok(int device) {
+int device_arithmetic_ok(int device) {
   const bool cached = device >= 0 && device < 64;
-  if (cached && g_div_state[device].load() == 1) return AUSM_OK;
   long long bad = 0;
-  if (!cached || g_div_state[device].load() == 0) {
-    const int r = division_check(&bad);
+  if (cached && g_arith_run[device].load()) {
+    bad = g_arith_bad[device].load();
+  } else {
+    const int r = arithmetic_check(&bad);
     if (r != AUSM_OK) return r;
-    if (cached) g_div_state[device].store(bad == 0 ? 1 : 2);
-    if (bad == 0) return AUSM_OK;
+    if (cached) {
+      g_arith_bad[device].store(bad);
+      g_arith_run[device].store(true);
+    }
   }
+  if (bad == 0) return AUSM_OK;
   return set_error(AUSM_ERR_HIP,
-                   "device %d: the refined reciprocal is not the correctly rounded 1/d for every "
-                   "divisor, which the exact arithmetic relies on", device);
+                   "device %d: the short division or square root of the exact arithmetic differs "
+                   "from IEEE on %lld inputs (ausm_check_arithmetic)", device, bad);
 }
 
 // Host-side parameter block -> device copy (the Uniforms upload,
@@ -333,7 +339,7 @@ int ausm_ctx_create(int device, int M, int N, ausm_ctx** out) {
   if (device < 0 || device >= ndev)
     return set_error(AUSM_ERR_ARG, "device %d not present (%d devices)", device, ndev);
   CHECK_HIP(hipSetDevice(device), "hipSetDevice");
-  const int dr = device_division_ok(device);
+  const int dr = device_arithmetic_ok(device);
   if (dr != AUSM_OK) return dr;
   ausm_ctx* c = new ausm_ctx();
   c->device = device;
@@ -1210,12 +1216,12 @@ int ausm_profile_stages(ausm_ctx* c, int reps, float* ms3) {
 
 }  // extern "C"
 
-int ausm_check_division(int device, long long* mismatches) {
+int ausm_check_arithmetic(int device, long long* mismatches) {
   if (!mismatches) return set_error(AUSM_ERR_ARG, "mismatches is NULL");
   int ndev = 0;
   CHECK_HIP(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
   if (device < 0 || device >= ndev)
     return set_error(AUSM_ERR_ARG, "device %d not present (%d devices)", device, ndev);
   CHECK_HIP(hipSetDevice(device), "hipSetDevice");
-  return division_check(mismatches);
+  return arithmetic_check(mismatches);
 }

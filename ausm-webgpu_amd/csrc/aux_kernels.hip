@@ -565,7 +565,7 @@ template hipError_t launch_convert_planes<double, float>(const double*, float*, 
                                                          hipStream_t);
 
 // --------------------------------------------------------------------------
-// self-check of the exact arithmetic's refined reciprocal
+// self-check of the exact arithmetic's short division and square root
 // --------------------------------------------------------------------------
 // One thread per FP32 divisor d = 2^(e-127) * 1.m, e = 1 + blockIdx.y
 // (d in [2^-126, 2^126)), m = blockIdx.x * 256 + threadIdx.x: recip_rn(d)
@@ -581,9 +581,25 @@ __global__ __launch_bounds__(256) void division_check_kernel(unsigned long long*
   const unsigned long long w = __ballot(__float_as_uint(y) != __float_as_uint(ref));
   if ((threadIdx.x & 63) == 0 && w != 0ull) atomicAdd(bad, (unsigned long long)__popcll(w));
 }
-hipError_t launch_division_check(unsigned long long* bad, hipStream_t s) {
+// sqrt_rn(x) against the compiler's IEEE square root for every FP32 x in
+// [2^-96, 2^128) (biased exponents 31 .. 254) and for x = +0
+constexpr int kSqrtCheckFirstExponent = 31;
+constexpr int kSqrtCheckExponents = 254 - kSqrtCheckFirstExponent + 1;
+__global__ __launch_bounds__(256) void sqrt_check_kernel(unsigned long long* bad) {
+  const unsigned m = blockIdx.x * 256u + threadIdx.x;
+  const unsigned e = kSqrtCheckFirstExponent + blockIdx.y;
+  float x = __uint_as_float((e << 23) | m);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) x = 0.0f;
+  const unsigned long long w = __ballot(__float_as_uint(sqrt_rn(x)) != __float_as_uint(sqrtf(x)));
+  if ((threadIdx.x & 63) == 0 && w != 0ull) atomicAdd(bad, (unsigned long long)__popcll(w));
+}
+hipError_t launch_arithmetic_check(unsigned long long* bad, hipStream_t s) {
   hipLaunchKernelGGL(division_check_kernel, dim3((1u << 23) / 256u, kDivCheckExponents), dim3(256), 0,
                      s, bad);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sqrt_check_kernel, dim3((1u << 23) / 256u, kSqrtCheckExponents), dim3(256), 0, s,
+                     bad);
   return hipGetLastError();
 }
 

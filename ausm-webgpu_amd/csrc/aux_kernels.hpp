@@ -49,8 +49,10 @@ hipError_t launch_comp_to_f64(const float* hi, const float* lo, double* q, int M
 hipError_t launch_f64_to_comp(const double* q, float* hi, float* lo, int M, int N, int P, int PS,
                               hipStream_t s);
 
-// recip_rn(d) against the IEEE 1/d for every FP32 divisor in [2^-126, 2^126);
-// adds the number of mismatches to *bad (which the caller clears)
-hipError_t launch_division_check(unsigned long long* bad, hipStream_t s);
+// recip_rn(d) against the IEEE 1/d for every FP32 divisor in [2^-126, 2^126)
+// and sqrt_rn(x) against the IEEE square root for x = +0 and every FP32 x in
+// [2^-96, 2^128); adds the number of mismatches to *bad (which the caller
+// clears)
+hipError_t launch_arithmetic_check(unsigned long long* bad, hipStream_t s);
 
 }  // namespace ausm

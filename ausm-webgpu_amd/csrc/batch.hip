@@ -579,7 +579,7 @@ int ausm_batch_create(int device, int max_solves, size_t budget, ausm_batch** ou
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
     return set_error(AUSM_ERR_ARG, "device %d not present", device);
   if (hipSetDevice(device) != hipSuccess) return set_error(AUSM_ERR_HIP, "hipSetDevice failed");
-  const int dr = device_division_ok(device);
+  const int dr = device_arithmetic_ok(device);
   if (dr != AUSM_OK) return dr;
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return set_error(AUSM_ERR_HIP, "hipMemGetInfo failed");

@@ -66,8 +66,8 @@ struct ausm_ctx {
 namespace ausm {
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
-// AUSM_OK once the device's refined reciprocal has passed the exact-math check
-int device_division_ok(int device);
+// AUSM_OK once the device has passed the exact arithmetic's self-check
+int device_arithmetic_ok(int device);
 size_t round_up(size_t x, size_t a);
 // the FP32 parameter block of a free stream (inState formed in double,
 // stored as FP32, as the reference's host does)

@@ -67,7 +67,7 @@ using Flux = FluxT<float>;
 //   y = rcp(d);  y += y (1 - d y);  q = n y;  q += y (n - d q)
 // The refined reciprocal y = recip_rn(d) is the correctly rounded 1/d for
 // every normal d < 2^126 on gfx950: v_rcp_f32 followed by that Newton step
-// was checked for every FP32 divisor in the range (ausm_check_division runs
+// was checked for every FP32 divisor in the range (ausm_check_arithmetic runs
 // the check on the device; contexts and batches refuse a device that fails
 // it).  With y = RN(1/d), a quotient q = RN(n y) within an ulp and an exact
 // residual n - d q, one correction rounds correctly (Markstein's theorem).
@@ -105,6 +105,33 @@ __device__ __forceinline__ R div_by(R n, R d, R y) {
 template <typename R>
 __device__ __forceinline__ R div_rn(R n, R d) {
   return div_by(n, d, recip_rn(d));
+}
+
+// IEEE FP32 square root in fewer instructions, for x = +0 or
+// 2^-96 <= x < 2^128.  The compiler's correctly rounded square root is
+// v_sqrt_f32 (within one ulp, not correctly rounded: about 15 % of inputs
+// differ) followed by a choice among the result and its two neighbours by
+// the signs of their FMA residuals x - s' s, wrapped in a scaling of inputs
+// below 2^-96 and a select that passes zeros, infinities and NaN through:
+// 15 instructions.  sqrt_rn is the same choice without the wrapping, 9;
+// on its range the two are the same operations on the same values, and
+// ausm_check_arithmetic compares them on the device for every FP32 input of
+// the range.  Call sites state why their argument is in range; square roots
+// of speeds (which vanish in still air) keep the compiler's.  R = double:
+// the compiler's.
+template <typename R>
+__device__ __forceinline__ R sqrt_rn(R x) {
+  if constexpr (sizeof(R) == 4) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __int_as_float(__float_as_int(s) - 1);
+    const float up = __int_as_float(__float_as_int(s) + 1);
+    const float rdn = fmaf(-dn, s, x);
+    const float rup = fmaf(-up, s, x);
+    const float r = rdn <= 0.0f ? dn : s;
+    return rup > 0.0f ? up : r;
+  } else {
+    return sqrt(x);
+  }
 }
 
 template <typename R>
@@ -145,7 +172,9 @@ __device__ __forceinline__ Prim to_prim_fast(Cons q, float gm1) {
 // (the face length the residual uses, shaders/grid.js:143-146)
 template <typename R>
 __device__ __forceinline__ void normal_exact(R dx, R dy, R& nx, R& ny, R& len) {
-  len = sqrt(dx * dx + dy * dy);
+  // an edge of a grid cell, at least 2^-48 long (dx^2 + dy^2 >= 2^-96)
+  // unless the cell is degenerate
+  len = sqrt_rn(dx * dx + dy * dy);
   const R tx = dx / len, ty = dy / len;
   nx = -ty;
   ny = tx;
@@ -315,7 +344,8 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     const Real M_inf2 = div_rn(P.vx * P.vx + P.vy * P.vy, div_rn(gamma * P.inP, L.r));
     const Real sigma = Real(1.0);
     const Real h_t = Real(0.5) * (h_L + h_R);
-    const Real aStar = sqrt(fmax(Real(1e-6), ((Real(2.0) * gammaMinus1) / (gamma + Real(1.0))) * h_t));
+    // argument at least 1e-6
+    const Real aStar = sqrt_rn(fmax(Real(1e-6), ((Real(2.0) * gammaMinus1) / (gamma + Real(1.0))) * h_t));
     // aStar >= 1e-3, and the divisors are at least aStar
     const Real aHat_L = div_rn(aStar * aStar, fmax(u_L, aStar));
     const Real aHat_R = div_rn(aStar * aStar, fmax(-u_R, aStar));
@@ -357,8 +387,10 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
   } else {
     // ausm.js:348-384
     // numerators at least 1.4e-10, divisors at least 1e-10
-    const Real a_L = sqrt(div_rn(gamma * fmax(L.p, Real(1e-10)), fmax(L.r, Real(1e-10))));
-    const Real a_R = sqrt(div_rn(gamma * fmax(R.p, Real(1e-10)), fmax(R.r, Real(1e-10))));
+    // square roots of at least 1.4e-10 / rho: in sqrt_rn's range for any
+    // density below 2^60
+    const Real a_L = sqrt_rn(div_rn(gamma * fmax(L.p, Real(1e-10)), fmax(L.r, Real(1e-10))));
+    const Real a_R = sqrt_rn(div_rn(gamma * fmax(R.p, Real(1e-10)), fmax(R.r, Real(1e-10))));
     const Real a_i = (a_L + a_R) * Real(0.5);
     const Real M_L = u_L / a_i;
     const Real M_R = u_R / a_i;

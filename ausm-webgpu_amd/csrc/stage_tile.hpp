@@ -131,9 +131,10 @@ __device__ __forceinline__ R cell_wave_speed(const DevParamsT<R>& P, R rho, R mu
     const R vv = vx * vx + vy * vy;
     const R p = (e - (R(0.5) * rho) * vv) * (P.gamma - R(1));
     // p and rho of a physical state, and (|u| + c) S >= c S > 0 over twice
-    // the cell area, are inside div_rn's range
+    // the cell area, are inside div_rn's range, and gamma p / rho inside
+    // sqrt_rn's
     const R T = div_rn(p, rho);
-    const R c = sqrt(P.gamma * T);
+    const R c = sqrt_rn(P.gamma * T);
     const R speed = sqrt(vv);
     return div_rn((speed + c) * S, R(2) * area);
   }

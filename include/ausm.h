@@ -268,13 +268,15 @@ int ausm_get_state64(ausm_ctx* ctx, double* q);
 int ausm_eval_residual64(ausm_ctx* ctx, double* res);
 
 /* Self-check of the device arithmetic exact math relies on, no counterpart
- * in the reference: the exact path divides with a refined hardware
- * reciprocal that must be the correctly rounded 1/d.  Runs the check for
- * every FP32 divisor in [2^-126, 2^126) on `device` and returns the number
- * of divisors whose refined reciprocal differs (0 on gfx950).  Context and
- * batch creation run it once per device and refuse a device that fails;
+ * in the reference.  The exact path divides with a refined hardware
+ * reciprocal that must be the correctly rounded 1/d, and takes square roots
+ * with the IEEE sequence stripped of its special-range handling.  Runs both
+ * on `device` for every FP32 input of their ranges (divisors in
+ * [2^-126, 2^126); square roots of +0 and of [2^-96, 2^128)) and returns the
+ * number of inputs whose result differs from IEEE (0 on gfx950).  Context
+ * and batch creation run it once per device and refuse a device that fails;
  * callers of ausm_step_noalloc can run it once at set-up. */
-int ausm_check_division(int device, long long* mismatches);
+int ausm_check_arithmetic(int device, long long* mismatches);
 
 /* Timing hook, no counterpart in the reference (it times whole passes with
  * timestamp queries, main.js:597-602): launches each of the three stage

@@ -10,7 +10,7 @@ with every operation rounded once to FP32 (round to nearest, ties to even).
 It claims the IEEE quotient RN(n / d) bit for bit for d > 0 normal with
 d < 2^126, n = +0 or 2^-100 <= |n|, |n / d| < 2^96, given the premise that
 the refined reciprocal y is the correctly rounded 1/d.  The premise is a
-property of the hardware reciprocal: ausm_check_division verifies it on
+property of the hardware reciprocal: ausm_check_arithmetic verifies it on
 the device for every FP32 divisor (tests/test_gpu_parity.py), and context
 and batch creation refuse a device that fails it.  Here, with y = RN(1/d),
 each operation is evaluated exactly and rounded to FP32 by hand, over

@@ -93,13 +93,15 @@ def max_rel(a, b):
 
 
 # --------------------------------------------------------------------------
-def test_refined_reciprocal_is_correctly_rounded():
-    """The exact arithmetic's premise on this device: v_rcp_f32 plus one
-    Newton step is the IEEE 1/d for every FP32 divisor in [2^-126, 2^126)."""
+def test_exact_arithmetic_premises_on_this_device():
+    """The exact arithmetic's premises on this device: v_rcp_f32 plus one
+    Newton step is the IEEE 1/d for every FP32 divisor in [2^-126, 2^126),
+    and the short square root equals the IEEE one for +0 and every FP32
+    input in [2^-96, 2^128)."""
     import ctypes
 
     bad = ctypes.c_longlong(-1)
-    ausm.lib.check(ausm.lib.load().ausm_check_division(0, ctypes.byref(bad)))
+    ausm.lib.check(ausm.lib.load().ausm_check_arithmetic(0, ctypes.byref(bad)))
     assert bad.value == 0
 
 
