@@ -72,8 +72,8 @@ using Flux = FluxT<float>;
 // it).  With y = RN(1/d), a quotient q = RN(n y) within an ulp and an exact
 // residual n - d q, one correction rounds correctly (Markstein's theorem).
 // The residual is exact, and the result the IEEE quotient bit for bit,
-// whenever d > 0 is normal with d < 2^126, n = +0 or 2^-100 <= |n|, and
-// |n / d| < 2^96.  div_by takes the refined reciprocal, so that quotients
+// whenever d > 0 is normal with d < 2^126, and n = +0 or 2^-100 <= |n| with
+// 2^-125 <= |n / d| < 2^96.  div_by takes the refined reciprocal, so that quotients
 // with one divisor share it.  Every call site states why its operands are
 // in range; elsewhere the exact path keeps the compiler's division
 // (quotients whose numerators can be arbitrarily small, such as momenta and
@@ -105,6 +105,38 @@ __device__ __forceinline__ R div_by(R n, R d, R y) {
 template <typename R>
 __device__ __forceinline__ R div_rn(R n, R d) {
   return div_by(n, d, recip_rn(d));
+}
+
+// Quotients n[k] / d of numerators that may be arbitrarily small (momenta,
+// velocities and residuals, which fall through the denormal range at the
+// front of a disturbance spreading into a uniform flow) by one divisor d > 0
+// with 2^-126 <= d < 2^26, the IEEE quotients bit for bit: while no lane
+// holds a nonzero numerator below 2^-96 in magnitude (frexp exponent above
+// -96), every quotient is div_by on one refined reciprocal (in div_rn's
+// range: |n / d| > 2^-122) with the sign of a zero numerator kept, as IEEE
+// keeps it (+-0 / d = +-0); otherwise the wavefront takes the compiler's
+// division.  The test is wave-uniform, so the two paths never diverge
+// within a wavefront; lanes with live = false (halo lanes whose quotients
+// are not used) do not take part in it.  R = double: the compiler's
+// division.
+template <int K, typename R>
+__device__ __forceinline__ void div_free(const R (&n)[K], R d, R (&q)[K], bool live = true) {
+  if constexpr (sizeof(R) == 4) {
+    int e = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) e = min(e, __builtin_amdgcn_frexp_expf(n[k]));
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(live && e <= -96) == 0, 1)) {
+      const float y = recip_rn(d);
+#pragma unroll
+      for (int k = 0; k < K; ++k) q[k] = __builtin_copysignf(div_by(n[k], d, y), n[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) q[k] = n[k] / d;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) q[k] = n[k] / d;
+  }
 }
 
 // IEEE FP32 square root in fewer instructions, for x = +0 or
@@ -148,8 +180,12 @@ template <typename R>
 __device__ __forceinline__ PrimT<R> to_prim_exact(ConsT<R> q, R gamma) {
   PrimT<R> w;
   w.r = q.r;
-  w.u = q.mu / q.r;
-  w.v = q.mv / q.r;
+  // the density of a physical state is inside div_free's divisor range
+  const R mom[2] = {q.mu, q.mv};
+  R vel[2];
+  div_free(mom, q.r, vel);
+  w.u = vel[0];
+  w.v = vel[1];
   const R d = w.u * w.u + w.v * w.v;
   const R p = (q.e - (R(0.5) * q.r) * d) * (gamma - R(1.0));
   w.p = fmax(R(1e-7), p);
@@ -175,7 +211,10 @@ __device__ __forceinline__ void normal_exact(R dx, R dy, R& nx, R& ny, R& len) {
   // an edge of a grid cell, at least 2^-48 long (dx^2 + dy^2 >= 2^-96)
   // unless the cell is degenerate
   len = sqrt_rn(dx * dx + dy * dy);
-  const R tx = dx / len, ty = dy / len;
+  const R d[2] = {dx, dy};
+  R t[2];
+  div_free(d, len, t);  // a grid edge's length
+  const R tx = t[0], ty = t[1];
   nx = -ty;
   ny = tx;
 }
@@ -350,10 +389,18 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     const Real aHat_L = div_rn(aStar * aStar, fmax(u_L, aStar));
     const Real aHat_R = div_rn(aStar * aStar, fmax(-u_R, aStar));
     const Real a_i = fmin(aHat_L, aHat_R);
-    const Real M_L = u_L / a_i;
-    const Real M_R = u_R / a_i;
+    // a_i >= a*^2 / max(|u|, a*) is a sound speed of the state: inside
+    // div_free's divisor range, and so are 2 a_i^2 and rho_i a_i^2 below
+    const Real un[2] = {u_L, u_R};
+    Real Mn[2];
+    div_free(un, a_i, Mn);
+    const Real M_L = Mn[0];
+    const Real M_R = Mn[1];
     const Real a_i2 = a_i * a_i;
-    const Real Mbar2 = (u_L * u_L + u_R * u_R) / (a_i2 * Real(2.0));
+    const Real mb[1] = {u_L * u_L + u_R * u_R};
+    Real mq[1];
+    div_free(mb, a_i2 * Real(2.0), mq);
+    const Real Mbar2 = mq[0];
     const Real M_o2 = fmin(Real(1.0), fmax(Mbar2, M_inf2));
     const Real f_a = M_o2 * (Real(2.0) - M_o2);
     const Real rho_i = Real(0.5) * (L.r + R.r);
@@ -369,7 +416,10 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     Real kp = P.Kp / f_a;
     kp = kp * fmax(Real(1.0) - sigma * Mbar2, Real(0.0));
     kp = kp * (R.p - L.p);
-    kp = kp / (rho_i * a_i2);
+    const Real kn[1] = {kp};
+    Real kq[1];
+    div_free(kn, rho_i * a_i2, kq);
+    kp = kq[0];
     const Real M_i = (M4L + M4R) - kp;
     const Real alpha4 = Real(0.75) * (-Real(4.0) + (Real(5.0) * f_a) * f_a);
     const Real P5subL = M2pL * ((Real(2.0) - M_L) + (alpha4 * M_L) * M2mL);
@@ -392,10 +442,14 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     const Real a_L = sqrt_rn(div_rn(gamma * fmax(L.p, Real(1e-10)), fmax(L.r, Real(1e-10))));
     const Real a_R = sqrt_rn(div_rn(gamma * fmax(R.p, Real(1e-10)), fmax(R.r, Real(1e-10))));
     const Real a_i = (a_L + a_R) * Real(0.5);
-    const Real M_L = u_L / a_i;
-    const Real M_R = u_R / a_i;
     const Real Uarc = sqrt(Real(0.5) * (vvL + vvR));
-    const Real Marc = fmin(Real(1.0), Uarc / a_i);
+    // a_i, the mean sound speed, is inside div_free's divisor range
+    const Real sn[3] = {u_L, u_R, Uarc};
+    Real sq[3];
+    div_free(sn, a_i, sq);
+    const Real M_L = sq[0];
+    const Real M_R = sq[1];
+    const Real Marc = fmin(Real(1.0), sq[2]);
     const Real X = (Real(1.0) - Marc) * (Real(1.0) - Marc);
     const Real qL = M_L + Real(1.0), qR = M_R - Real(1.0);
     const Real sqL = (Real(0.5) * qL) * qL, sqR = (Real(0.5) * qR) * qR;
@@ -413,7 +467,10 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
       P_i = Real(0.5) * (t1 + t2);
     }
     const Real delta_p = R.p - L.p;
-    const Real ubar_n = (L.r * fabs(u_L) + R.r * fabs(u_R)) / (L.r + R.r);
+    const Real ubn[1] = {L.r * fabs(u_L) + R.r * fabs(u_R)};
+    Real ubq[1];
+    div_free(ubn, L.r + R.r, ubq);
+    const Real ubar_n = ubq[0];
     const Real g = wsat(-M_L) * wsat(M_R);
     const Real ubp = wmix(ubar_n, fabs(u_L), g);
     const Real ubm = wmix(ubar_n, fabs(u_R), g);

@@ -472,10 +472,12 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
         const R ns[4] = {-(((PR.f0 - PX.f0) + Gu.f0) - Gd.f0), -(((PR.f1 - PX.f1) + Gu.f1) - Gd.f1),
                          -(((PR.f2 - PX.f2) + Gu.f2) - Gd.f2), -(((PR.f3 - PX.f3) + Gu.f3) - Gd.f3)};
         R ia = R(1);
+        R rq[4];  // exact math: the four quotients ns / A (a cell area)
         if constexpr (FAST) ia = __builtin_amdgcn_rcpf(area);
+        else div_free(ns, area, rq, owned);
         auto res = [&](int k) -> R {
           if constexpr (FAST) return ns[k] * ia;
-          else return ns[k] / area;
+          else return rq[k];
         };
 
         if (owned) {
@@ -561,7 +563,9 @@ __device__ __forceinline__ R stage_tile(const DevParamsT<R>& P, const StageArgsT
               if constexpr (FAST) {
                 for (int k = 0; k < 4; ++k) nq[k] = fmaf(2.0f, add_inc(k, cq[k]), qn[k]) * (1.0f / 3.0f);
               } else {
-                for (int k = 0; k < 4; ++k) nq[k] = (qn[k] + R(2) * add_inc(k, cq[k])) / R(3);
+                R t3[4];
+                for (int k = 0; k < 4; ++k) t3[k] = qn[k] + R(2) * add_inc(k, cq[k]);
+                div_free(t3, R(3), nq);
               }
             }
           }

@@ -190,6 +190,36 @@ def test_rest_start_bitwise():
 
 
 @pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("amp", [1e-33, 1e-20])
+def test_tiny_momenta_exact_bitwise(scheme, amp):
+    """Air at rest with a velocity ripple of amplitude amp on one side of the
+    ring: the momenta, velocities and residuals there are below 2^-96 (amp
+    1e-33, the exact path's fallback to the compiler's division) or above it
+    (1e-20, the short division), beside cells with zero momentum."""
+    M, N = 96, 24
+    vtx, _, _ = make_grid(M, N, code=12, iters=80, radius=6.0)
+    # a Mach 0.3 free stream beyond the far field (AUSM+-up's M_inf^2 > 0)
+    s, ref, par = new_pair(vtx, 0.3, 0.0, scheme, VANALBADA, cfl=0.5)
+    jj, ii = np.meshgrid(np.arange(N + 3), np.arange(M), indexing="ij")
+    rip = amp * np.sin(2 * math.pi * ii / M) * np.exp(-0.1 * jj) * (ii < M // 2)
+    st = np.zeros((N + 3, M, 4), dtype=np.float32)
+    st[..., 0] = 1.0
+    st[..., 1] = rip.astype(np.float32)
+    st[..., 2] = (-0.5 * rip).astype(np.float32)
+    st[..., 3] = np.float32(1.0 / 1.4 / 0.4)
+    s.set_state(st)
+    ref.set_state(st)
+    r_gpu = s.residual()
+    r_ref = Oracle(32).residual(par, scheme, VANALBADA, vtx, st)
+    assert np.array_equal(r_gpu, r_ref), max_rel(r_gpu, r_ref)
+    s.step(6)
+    ref.step(6)
+    got = s.state()
+    assert np.array_equal(got[1:N + 1], ref.state[1:N + 1]), max_rel(got, ref.state)
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
 def test_shock_state_exact_bitwise(scheme):
     """A shock-containing state from a supersonic run, then exact steps."""
     M, N = 128, 48
