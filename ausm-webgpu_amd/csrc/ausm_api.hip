@@ -961,24 +961,30 @@ int ausm_prepare_steps(ausm_ctx* c, int nsteps) {
   const int ty = current_rows(c);
   const int K = c->refresh;
   const int G = chunk_steps(K);
-  // the chunk sequence ausm_step would run from the current counters
-  int since = c->since_refresh, valid = c->n_ws_valid;
-  bool ws_fresh = c->ws_fresh;
-  int left = nsteps;
-  while (left > 0) {
-    if (since == 0 && ws_fresh && left >= K) {
-      const int len = left >= G ? G : left / K * K;
-      GraphEntry* ge = nullptr;
-      const int r = graph_for(c, ty, len, valid, &ge);
-      if (r != AUSM_OK) return r;
-      left -= len;
-      valid = stage_blocks(c->M, c->N, ty);
-    } else {
-      const bool refresh = since + 1 >= K;
-      ws_fresh = refresh;
-      since = refresh ? 0 : since + 1;
-      if (refresh) valid = stage_blocks(c->M, c->N, ty);
-      left -= 1;
+  const int steady = stage_blocks(c->M, c->N, ty);
+  // the chunk sequence ausm_step would run from the current counters, and
+  // from a refresh boundary left by an earlier step (what a call after
+  // other steps starts from)
+  for (int from = 0; from < 2; ++from) {
+    if (from == 1 && c->since_refresh == 0 && c->ws_fresh && c->n_ws_valid == steady) break;
+    int since = from ? 0 : c->since_refresh, valid = from ? steady : c->n_ws_valid;
+    bool ws_fresh = from ? true : c->ws_fresh;
+    int left = nsteps;
+    while (left > 0) {
+      if (since == 0 && ws_fresh && left >= K) {
+        const int len = left >= G ? G : left / K * K;
+        GraphEntry* ge = nullptr;
+        const int r = graph_for(c, ty, len, valid, &ge);
+        if (r != AUSM_OK) return r;
+        left -= len;
+        valid = steady;
+      } else {
+        const bool refresh = since + 1 >= K;
+        ws_fresh = refresh;
+        since = refresh ? 0 : since + 1;
+        if (refresh) valid = steady;
+        left -= 1;
+      }
     }
   }
   return AUSM_OK;

@@ -225,10 +225,11 @@ int ausm_set_state(ausm_ctx* ctx, const float* q);
 /* interior rows as stored; ghost rows as the next boundary pass makes them */
 int ausm_get_state(ausm_ctx* ctx, float* q);
 int ausm_step(ausm_ctx* ctx, int nsteps, ausm_step_info* info);
-/* Captures and instantiates the step graphs that a following
- * ausm_step(ctx, nsteps, ...) will launch, without running them; the
- * graphs stay cached on the context until a setting, the grid or the state
- * changes.  No counterpart in the reference (its command encoder records a
+/* Captures, instantiates and uploads the step graphs that a following
+ * ausm_step(ctx, nsteps, ...) will launch -- from the current point, and
+ * from the refresh boundary any earlier step call leaves -- without running
+ * them; the graphs stay cached on the context until a setting, the grid or
+ * the state changes.  No counterpart in the reference (its command encoder records a
  * frame's passes every frame): lets a caller keep graph set-up out of a
  * timed or latency-critical step call. */
 int ausm_prepare_steps(ausm_ctx* ctx, int nsteps);
