@@ -174,6 +174,19 @@ template <typename R>
 __device__ __forceinline__ R wsat(R x) { return fmin(fmax(x, R(0)), R(1)); }
 template <typename R>
 __device__ __forceinline__ R wmix(R a, R b, R t) { return a * (R(1) - t) + b * t; }
+// mix(a, b, t) with t a step (0 or 1) at the call sites of the split
+// functions below: in FP32 the select t ? b : a, which equals
+// a (1 - t) + b t bit for bit whenever a and b are finite and the operand
+// kept is not a -0 beside a +0 product of the other -- at these sites the
+// subsonic polynomial is never -0 (its factors are bounded away from zero
+// on |M| < 1) and the supersonic operand is never -0.  The equality is
+// checked by brute force in FP32 on the operands these sites form
+// (tests/test_exact_select.py).  FP64 keeps the product form.
+template <typename R>
+__device__ __forceinline__ R wsel(R a, R b, R t) {
+  if constexpr (sizeof(R) == 4) return t != R(0) ? b : a;
+  else return wmix(a, b, t);
+}
 
 // toPrimitive (ausm.js:145-150)
 template <typename R>
@@ -411,8 +424,8 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     const Real M4subL = M2pL * (Real(1.0) + Real(2.0) * M2mL);
     const Real M4subR = M2mR * (Real(1.0) - Real(2.0) * M2pR);
     const Real supL = wstep(Real(1.0), fabs(M_L)), supR = wstep(Real(1.0), fabs(M_R));
-    const Real M4L = wmix(M4subL, fmax(Real(0.0), M_L), supL);
-    const Real M4R = wmix(M4subR, fmin(Real(0.0), M_R), supR);
+    const Real M4L = wsel(M4subL, fmax(Real(0.0), M_L), supL);
+    const Real M4R = wsel(M4subR, fmin(Real(0.0), M_R), supR);
     Real kp = P.Kp / f_a;
     kp = kp * fmax(Real(1.0) - sigma * Mbar2, Real(0.0));
     kp = kp * (R.p - L.p);
@@ -424,8 +437,8 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     const Real alpha4 = Real(0.75) * (-Real(4.0) + (Real(5.0) * f_a) * f_a);
     const Real P5subL = M2pL * ((Real(2.0) - M_L) + (alpha4 * M_L) * M2mL);
     const Real P5subR = M2mR * ((-Real(2.0) - M_R) - (alpha4 * M_R) * M2pR);
-    const Real P5L = wmix(P5subL, wstep(Real(0.0), M_L), supL);
-    const Real P5R = wmix(P5subR, wstep(Real(0.0), -M_R), supR);
+    const Real P5L = wsel(P5subL, wstep(Real(0.0), M_L), supL);
+    const Real P5R = wsel(P5subR, wstep(Real(0.0), -M_R), supR);
     Real ku = P.Ku * P5L;
     ku = ku * P5R;
     ku = ku * (L.r + R.r);
@@ -454,9 +467,9 @@ __device__ __forceinline__ FluxT<Real> face_flux_exact(const DevParamsT<Real>& P
     const Real qL = M_L + Real(1.0), qR = M_R - Real(1.0);
     const Real sqL = (Real(0.5) * qL) * qL, sqR = (Real(0.5) * qR) * qR;
     const Real supL = wstep(Real(1.0), fabs(M_L)), supR = wstep(Real(1.0), fabs(M_R));
-    const Real fpp = wsat(Real(0.5) * wmix(sqL * (Real(2.0) - Real(1.0) * M_L), Real(1.0) + Real(1.0) * wsign(M_L), supL));
+    const Real fpp = wsat(Real(0.5) * wsel(sqL * (Real(2.0) - Real(1.0) * M_L), Real(1.0) + Real(1.0) * wsign(M_L), supL));
     const Real fpm =
-        wsat(Real(0.5) * wmix(sqR * (Real(2.0) - -Real(1.0) * M_R), Real(1.0) + -Real(1.0) * wsign(M_R), supR));
+        wsat(Real(0.5) * wsel(sqR * (Real(2.0) - -Real(1.0) * M_R), Real(1.0) + -Real(1.0) * wsign(M_R), supR));
     if constexpr (SCHEME == SCHEME_SLAU2) {
       const Real t1 = (fpp - fpm) * (L.p - R.p);
       const Real t2 = ((Uarc * ((fpp + fpm) - Real(1.0))) * (L.r + R.r)) * a_i;
