@@ -36,10 +36,10 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=
          "-Wno-unused-result", "-I" + INCLUDE, "-I" + CSRC]
 
 
-# per-file additions: the single-grid stage kernels are scheduled for
-# instruction-level parallelism (1-3 % faster across sizes, 7 % for exact
-# math); the persistent batched kernel is not (3 % slower with it)
-EXTRA_FLAGS = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in STAGE_SOURCES}
+# per-file additions: the stage kernels (single-grid and batched) are
+# scheduled for instruction-level parallelism (single-grid 1-3 % faster
+# across sizes, 7 % for exact math; the 64-solve batched mix 1.5 %)
+EXTRA_FLAGS = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in STAGE_SOURCES + BATCH_SOURCES}
 
 
 def _compile(src):

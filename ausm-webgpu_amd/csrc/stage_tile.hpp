@@ -127,7 +127,11 @@ __device__ __forceinline__ R cell_wave_speed(const DevParamsT<R>& P, R rho, R mu
     const float speed = __builtin_amdgcn_sqrtf(vv);
     return (speed + c) * S * __builtin_amdgcn_rcpf(2.0f * area);
   } else {
-    const R vx = mu / rho, vy = mv / rho;
+    // the density of a physical state is inside div_free's divisor range
+    const R mom[2] = {mu, mv};
+    R vel[2];
+    div_free(mom, rho, vel);
+    const R vx = vel[0], vy = vel[1];
     const R vv = vx * vx + vy * vy;
     const R p = (e - (R(0.5) * rho) * vv) * (P.gamma - R(1));
     // p and rho of a physical state, and (|u| + c) S >= c S > 0 over twice
