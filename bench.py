@@ -473,8 +473,8 @@ def main():
 
     other = "fast" if args.math == "exact" else "exact"
     # Supplementary single-grid figures first: they also bring the GPU to its
-    # sustained clocks, so that a short timed region (the driver's --steps 20
-    # --warmup 5) measures the steady step rather than a cold GPU.
+    # sustained clocks, so that a short timed region (--steps 20 --warmup 5)
+    # measures a warm GPU.
     # The same workload at the reference README's literal 510 x 381 (the
     # O-grid the reference's default grid settings give), for comparability
     literal = None
