@@ -65,23 +65,32 @@ static int arithmetic_check(long long* mismatches) {
 static std::atomic<long long> g_arith_bad[64] = {};
 static std::atomic<bool> g_arith_run[64] = {};
 
-int device_arithmetic_ok(int device) {
-  const bool cached = device >= 0 && device < 64;
-  long long bad = 0;
-  if (cached && g_arith_run[device].load()) {
-    bad = g_arith_bad[device].load();
-  } else {
-    const int r = arithmetic_check(&bad);
-    if (r != AUSM_OK) return r;
-    if (cached) {
-      g_arith_bad[device].store(bad);
-      g_arith_run[device].store(true);
-    }
-  }
+long long device_arithmetic_cached(int device) {
+  if (device < 0 || device >= 64 || !g_arith_run[device].load()) return -1;
+  return g_arith_bad[device].load();
+}
+
+void device_arithmetic_store(int device, long long bad) {
+  if (device < 0 || device >= 64) return;
+  g_arith_bad[device].store(bad);
+  g_arith_run[device].store(true);
+}
+
+int device_arithmetic_verdict(int device, long long bad) {
   if (bad == 0) return AUSM_OK;
   return set_error(AUSM_ERR_HIP,
                    "device %d: the short division or square root of the exact arithmetic differs "
                    "from IEEE on %lld inputs (ausm_check_arithmetic)", device, bad);
+}
+
+int device_arithmetic_ok(int device) {
+  long long bad = device_arithmetic_cached(device);
+  if (bad < 0) {
+    const int r = arithmetic_check(&bad);
+    if (r != AUSM_OK) return r;
+    device_arithmetic_store(device, bad);
+  }
+  return device_arithmetic_verdict(device, bad);
 }
 
 // Host-side parameter block -> device copy (the Uniforms upload,
@@ -1229,5 +1238,7 @@ int ausm_check_arithmetic(int device, long long* mismatches) {
   if (device < 0 || device >= ndev)
     return set_error(AUSM_ERR_ARG, "device %d not present (%d devices)", device, ndev);
   CHECK_HIP(hipSetDevice(device), "hipSetDevice");
-  return arithmetic_check(mismatches);
+  const int r = arithmetic_check(mismatches);
+  if (r == AUSM_OK) device_arithmetic_store(device, *mismatches);
+  return r;
 }

@@ -66,8 +66,14 @@ struct ausm_ctx {
 namespace ausm {
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
-// AUSM_OK once the device has passed the exact arithmetic's self-check
+// AUSM_OK once the device has passed the exact arithmetic's self-check (run
+// on the first call per device and process, with an allocation)
 int device_arithmetic_ok(int device);
+// the self-check's mismatch count on `device` in this process, -1 before it
+// has run; storing a count; the error (or AUSM_OK) for a count
+long long device_arithmetic_cached(int device);
+void device_arithmetic_store(int device, long long bad);
+int device_arithmetic_verdict(int device, long long bad);
 size_t round_up(size_t x, size_t a);
 // the FP32 parameter block of a free stream (inState formed in double,
 // stored as FP32, as the reference's host does)

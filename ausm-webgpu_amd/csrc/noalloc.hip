@@ -59,7 +59,7 @@ int scratch_address(int device, unsigned char** out) {
 // aligned to 256 B)
 struct ScratchLayout {
   int P, PS, tiles_x, max_parts;
-  size_t q[3], vx, vy, ws, res, scal_f, scal_d, par, total;
+  size_t q[3], vx, vy, ws, res, scal_f, scal_d, par, check, total;
 };
 
 ScratchLayout layout_for(int M, int N, int ty) {
@@ -82,6 +82,7 @@ ScratchLayout layout_for(int M, int N, int ty) {
   L.scal_f = take(sizeof(float) * 8);
   L.scal_d = take(sizeof(double) * 8);
   L.par = take(sizeof(DevParams));
+  L.check = take(sizeof(unsigned long long));
   L.total = off;
   return L;
 }
@@ -90,6 +91,10 @@ ScratchLayout layout_for(int M, int N, int ty) {
 // buffer has to outlive the call, and a captured graph holds no copy node)
 __global__ void put_params_kernel(DevParams p, DevParams* dst) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = p;
+}
+
+__global__ void zero_count_kernel(unsigned long long* n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *n = 0ull;
 }
 
 // vertices in the reference's layout ((N+1) x M x 2) -> two planes of pitch
@@ -161,17 +166,44 @@ extern "C" int ausm_step_noalloc(int device, const ausm_step_desc* d, const floa
   double* scal_d = (double*)(base + L.scal_d);
   DevParams* dpar = (DevParams*)(base + L.par);
 
+#define NA_CHECK(expr, what)                                  \
+  do {                                                        \
+    hipError_t _e = (expr);                                   \
+    if (_e != hipSuccess) return hip_fail(_e, what);          \
+  } while (0)
+  if (d->math == AUSM_MATH_EXACT) {
+    // exact math rests on the device's short division and square root:
+    // the first exact call on a device runs the self-check (as context and
+    // batch creation do), counting into `scratch`, and every later call
+    // consults its result; under stream capture the check cannot run
+    long long bad = device_arithmetic_cached(device);
+    if (bad < 0) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      NA_CHECK(hipStreamIsCapturing(s, &cs), "capture status");
+      if (cs != hipStreamCaptureStatusNone)
+        return set_error(AUSM_ERR_ORDER,
+                         "exact math on device %d: run ausm_check_arithmetic(%d) once before "
+                         "capturing allocation-free steps", device, device);
+      unsigned long long* cnt = (unsigned long long*)(base + L.check);
+      hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, s, cnt);
+      NA_CHECK(hipGetLastError(), "arithmetic check");
+      NA_CHECK(launch_arithmetic_check(cnt, s), "arithmetic check");
+      unsigned long long h = 0;
+      NA_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, s), "arithmetic check");
+      NA_CHECK(hipStreamSynchronize(s), "arithmetic check");
+      bad = (long long)h;
+      device_arithmetic_store(device, bad);
+    }
+    const int v = device_arithmetic_verdict(device, bad);
+    if (v != AUSM_OK) return v;
+  }
+
   DevParams hp{};
   make_freestream(hp, d->rho, d->p, d->vx, d->vy, d->gamma);
   hp.maxdt = d->maxdt;
   hp.cfl = d->cfl;
   hp.Kp = d->K_p;
   hp.Ku = d->K_u;
-#define NA_CHECK(expr, what)                                  \
-  do {                                                        \
-    hipError_t _e = (expr);                                   \
-    if (_e != hipSuccess) return hip_fail(_e, what);          \
-  } while (0)
   hipLaunchKernelGGL(put_params_kernel, dim3(1), dim3(64), 0, s, hp, dpar);
   NA_CHECK(hipGetLastError(), "parameters");
   hipLaunchKernelGGL(vertex_planes_kernel, dim3((L.P + 255) / 256, N + 1), dim3(256), 0, s,

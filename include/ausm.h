@@ -274,9 +274,12 @@ int ausm_eval_residual64(ausm_ctx* ctx, double* res);
  * with the IEEE sequence stripped of its special-range handling.  Runs both
  * on `device` for every FP32 input of their ranges (divisors in
  * [2^-126, 2^126); square roots of +0 and of [2^-96, 2^128)) and returns the
- * number of inputs whose result differs from IEEE (0 on gfx950).  Context
- * and batch creation run it once per device and refuse a device that fails;
- * callers of ausm_step_noalloc can run it once at set-up. */
+ * number of inputs whose result differs from IEEE (0 on gfx950).  The
+ * library keeps the count per device for the process: context and batch
+ * creation and the first exact-math ausm_step_noalloc call on a device run
+ * the check unless it has run, and all of them refuse a device that fails.
+ * Callers that capture exact-math ausm_step_noalloc calls into a graph run
+ * it once beforehand (the check cannot run under capture). */
 int ausm_check_arithmetic(int device, long long* mismatches);
 
 /* Timing hook, no counterpart in the reference (it times whole passes with
@@ -303,7 +306,10 @@ int ausm_profile_stages(ausm_ctx* ctx, int reps, float* ms3);
  * overlap on one GPU race.  Plain FP32 (the reference's arithmetic) only;
  * the first step takes its dt from the input state, later ones refresh it
  * every steps_per_dt_refresh steps.  info (optional) is read back with a
- * stream synchronisation; with info NULL the call only enqueues work. */
+ * stream synchronisation; with info NULL the call only enqueues work,
+ * except that the first exact-math call on a device runs the device's
+ * arithmetic self-check (ausm_check_arithmetic) on `stream` and waits for
+ * it, and refuses to (AUSM_ERR_ORDER) if the stream is capturing. */
 #define AUSM_NOALLOC_SCRATCH_MB 64
 typedef struct ausm_step_desc {
   int M, N;                   /* cells around / radial */

@@ -133,6 +133,80 @@ def test_noalloc_step_matches_context_step_twice(math_mode):
             assert np.array_equal(q[1:N + 1], ref.state[1:N + 1])
 
 
+FIRST_CALL = r"""
+import ctypes, os, sys
+sys.path.insert(0, os.environ["AUSM_TEST_DIR"])
+sys.path.insert(0, os.path.join(os.path.dirname(os.environ["AUSM_TEST_DIR"]), "ausm-webgpu_amd"))
+import numpy as np
+import ausm
+from ausm import lib as L
+from oracle_lib import Solver
+from test_zz_no_alloc import Hip, desc, grid, perturbed, make_params
+hip, lib = Hip(), ausm.load()
+M, N, steps = 64, 24, 9
+vtx = grid(M, N, 2412)
+fs = ausm.freestream(2.0, 0.0)
+st = perturbed(make_params(2.0, 0.0), M, N, seed=3)
+dv, _ = hip.upload(vtx)
+dq, shape = hip.upload(st)
+# a fast-math call first (no check needed): the library's kernels and its
+# scratch address are set up outside any capture
+dqf, _ = hip.upload(st)
+df = desc(M, N, fs, ausm.AUSMUP, ausm.VANALBADA, L.MATH_FAST, L.DT_GLOBAL, 1)
+L.check(lib.ausm_step_noalloc(0, ctypes.byref(df), dv, dqf, steps, None, None))
+assert hip.L.hipDeviceSynchronize() == 0
+hip.free(dqf)
+d = desc(M, N, fs, ausm.AUSMUP, ausm.VANALBADA, L.MATH_EXACT, L.DT_GLOBAL, 1)
+# no context, batch or check has run in this process: under stream capture
+# the first exact call is refused before it enqueues anything
+hip.L.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+hip.L.hipStreamBeginCapture.argtypes = [ctypes.c_void_p, ctypes.c_int]
+hip.L.hipStreamEndCapture.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+hip.L.hipGraphDestroy.argtypes = [ctypes.c_void_p]
+hip.L.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+stream, graph = ctypes.c_void_p(), ctypes.c_void_p()
+assert hip.L.hipStreamCreate(ctypes.byref(stream)) == 0
+assert hip.L.hipStreamBeginCapture(stream, 2) == 0  # relaxed mode
+rc = lib.ausm_step_noalloc(0, ctypes.byref(d), dv, dq, steps, stream, None)
+msg = lib.ausm_last_error().decode()
+assert hip.L.hipStreamEndCapture(stream, ctypes.byref(graph)) == 0
+if graph.value:
+    assert hip.L.hipGraphDestroy(graph) == 0
+assert hip.L.hipStreamDestroy(stream) == 0
+assert rc == -4 and "ausm_check_arithmetic" in msg, (rc, msg)
+# outside capture the first exact call runs the check itself and steps
+info = L.StepInfo()
+L.check(lib.ausm_step_noalloc(0, ctypes.byref(d), dv, dq, steps, None, ctypes.byref(info)))
+q = hip.download(dq, shape)
+ref = Solver(32, make_params(2.0, 0.0, cfl=0.5, maxdt=1.0), ausm.AUSMUP, ausm.VANALBADA, vtx)
+ref.set_state(st)
+ref.step(steps)
+assert np.array_equal(q[1:N + 1], ref.state[1:N + 1])
+mism = ctypes.c_longlong(-1)
+L.check(lib.ausm_check_arithmetic(0, ctypes.byref(mism)))
+assert mism.value == 0
+print("first-call check ok", flush=True)
+"""
+
+
+@pytest.mark.gpu
+def test_noalloc_first_exact_call_runs_the_device_check():
+    """In a process where no context or batch has run the arithmetic
+    self-check, the first exact-math allocation-free call runs it (with its
+    counter in the scratch array) and then steps bit-for-bit the oracle;
+    under stream capture it is refused instead, before enqueuing anything.
+    Needs a fresh process, so the case runs in a child."""
+    import os
+    import subprocess
+    import sys
+    env = dict(os.environ, AUSM_TEST_DIR=os.path.dirname(os.path.abspath(__file__)))
+    print("[noalloc] first exact call in a child process", flush=True)
+    r = subprocess.run([sys.executable, "-c", FIRST_CALL], env=env, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "first-call check ok" in r.stdout
+
+
 def test_noalloc_step_refuses_grids_beyond_its_scratch():
     """Checked before any GPU work: runs without a GPU."""
     lib = ausm.load()
