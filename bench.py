@@ -192,13 +192,31 @@ def cpu_baseline(vtx, M, N, target_s=15.0, q_gpu=None, r_gpu=None):
                      "exact_math_residual_bitwise_equal_to_ref32": bool(np.array_equal(r_gpu, r_ref)),
                      "max_abs_diff": float(np.abs(r_gpu.astype(np.float64) - r_ref).max())}
     return {"value": M * N * n / dt, "unit": "cell-steps/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "residual_agreement_vs_ref32": agreement,
             "sample": f"{n} steps of the 512x384 workload (ref32 oracle, 15 unfused passes per "
                       f"step, OpenMP over rows, {threads} threads, {dt:.1f} s)",
             "steps_per_s": n / dt,
             "single_thread": {"value": M * N * n1 / dt1, "unit": "cell-steps/s", "cores": 1,
                               "sample": f"{n1} steps, {dt1:.1f} s"}}
+
+
+def host_cpu():
+    """The host's CPU model and the logical CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
 def max_rel(a, ref):
