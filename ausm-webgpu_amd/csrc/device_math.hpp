@@ -106,6 +106,22 @@ template <typename R>
 __device__ __forceinline__ R div_rn(R n, R d) {
   return div_by(n, d, recip_rn(d));
 }
+// div_by with the residual formed as d q - n and subtracted in the
+// correction.  For n != 0 the residual is exact either way, so -(d q - n)
+// is n - d q and the quotient is div_by's bit for bit; for n = +-0 (q =
+// +-0) d q - n is +0 and q - 0 y keeps q's sign, so a zero numerator gives
+// IEEE's +-0 / d = +-0 where div_by's n - d q = +0 would turn -0 into +0.
+// Same instruction count as div_by (the negations are operand modifiers).
+template <typename R>
+__device__ __forceinline__ R div_by_zs(R n, R d, R y) {
+  if constexpr (sizeof(R) == 4) {
+    const float q = n * y;
+    return fmaf(-fmaf(d, q, -n), y, q);
+  } else {
+    (void)y;
+    return n / d;
+  }
+}
 
 // Quotients n[k] / d of numerators that may be arbitrarily small (momenta,
 // velocities and residuals, which fall through the denormal range at the
@@ -113,8 +129,8 @@ __device__ __forceinline__ R div_rn(R n, R d) {
 // with 2^-126 <= d < 2^26, the IEEE quotients bit for bit: while no lane
 // holds a nonzero numerator below 2^-96 in magnitude (frexp exponent above
 // -96), every quotient is div_by on one refined reciprocal (in div_rn's
-// range: |n / d| > 2^-122) with the sign of a zero numerator kept, as IEEE
-// keeps it (+-0 / d = +-0); otherwise the wavefront takes the compiler's
+// range: |n / d| > 2^-122) in the form that keeps the sign of a zero
+// numerator, as IEEE keeps it (+-0 / d = +-0); otherwise the wavefront takes the compiler's
 // division.  The test is wave-uniform, so the two paths never diverge
 // within a wavefront; lanes with live = false (halo lanes whose quotients
 // are not used) do not take part in it.  R = double: the compiler's
@@ -128,7 +144,7 @@ __device__ __forceinline__ void div_free(const R (&n)[K], R d, R (&q)[K], bool l
     if (__builtin_expect(__builtin_amdgcn_ballot_w64(live && e <= -96) == 0, 1)) {
       const float y = recip_rn(d);
 #pragma unroll
-      for (int k = 0; k < K; ++k) q[k] = __builtin_copysignf(div_by(n[k], d, y), n[k]);
+      for (int k = 0; k < K; ++k) q[k] = div_by_zs(n[k], d, y);
     } else {
 #pragma unroll
       for (int k = 0; k < K; ++k) q[k] = n[k] / d;

@@ -147,3 +147,41 @@ def test_the_cases_need_the_correctly_rounded_reciprocal():
         if any(seq(n, d, y) != want for y in off_by_one_ulp(d)):
             wrong += 1
     assert wrong > 0
+
+
+def seq_zs(n, d, y):
+    """div_by_zs: the residual formed as d q - n and subtracted"""
+    q = rn32(n * y)
+    return fma(-fma(d, q, -n), y, q)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_zero_sign_form_is_the_same_quotient(seed):
+    """For nonzero numerators the residual is exact in both forms, so
+    -(d q - n) = n - d q and div_by_zs is div_by bit for bit."""
+    for n, d in cases(seed, 1500):
+        y = recip_rn(d)
+        assert seq_zs(n, d, y) == seq(n, d, y) == rn32(n / d), (n, d)
+
+
+def _ieee_zero_steps(n, d, y, zs):
+    """The two forms on a zero numerator in binary floating point, where
+    every product and sum of the sequence is exact (so float64 gives the
+    FP32 results, signs of zero included)."""
+    q = n * y
+    if zs:
+        r = d * q + (-n)
+        return (-r) * y + q
+    r = (-d) * q + n
+    return r * y + q
+
+
+@pytest.mark.parametrize("d", [1.0, 3.0 / 7.0, 2.0 ** -100, 2.0 ** 25])
+def test_zero_sign_form_keeps_the_sign_of_zero(d):
+    """IEEE gives +-0 / d = +-0: div_by_zs keeps it; div_by's n - d q is +0
+    for both zeros and turns -0 into +0 (why div_free uses div_by_zs)."""
+    y = float(recip_rn(Fraction(d)))
+    for n in (0.0, -0.0):
+        out = _ieee_zero_steps(n, d, y, True)
+        assert out == 0.0 and math.copysign(1.0, out) == math.copysign(1.0, n)
+    assert math.copysign(1.0, _ieee_zero_steps(-0.0, d, y, False)) == 1.0
