@@ -43,7 +43,7 @@ sys.path.insert(0, os.path.join(ROOT, "ausm-webgpu_amd"))
 
 ALG_BYTES_PER_CELL_STEP = 152.0  # 8 x 16 B state streams + 3 x 8 B vertex
 HBM_PEAK_GBPS = 8000.0
-VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9  # FP32 lane-ops/s, no packed math
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9  # FP32 lane-ops/s, single issue, no packed math
 METRIC = "cell-steps/s & % HBM roofline; batched converged solves/s + p50/p99 latency"
 # steps timed for the supplementary single-grid figures (510x381, SLAU2):
 # whatever --steps is, long enough that launch and graph latencies amortise
@@ -594,8 +594,16 @@ def main():
                 "valu": (None if valu is None else
                          valu * 64 / (launch_s * VALU_PEAK_LANE_OPS)),
                 "valu_source": "SQ_INSTS_VALU per launch x 64 lanes / (launch time x 39.3e12 "
-                               "FP32 lane-ops/s: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, no "
-                               "packed math)",
+                               "FP32 lane-ops/s: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one "
+                               "wave64 instruction per 4 cycles per SIMD, no packed math; two "
+                               "plain FP32 instructions of two wavefronts can issue together, "
+                               "at most twice that)",
+                "valu_active_share": c.get("valu_active_share_of_wave_life"),
+                "valu_coissue_share": c.get("valu_coissue_share"),
+                "valu_active_source": "SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES (share of a "
+                                      "wavefront's life spent issuing VALU) and "
+                                      "SQ_ACTIVE_INST_VALU2 / SQ_ACTIVE_INST_VALU (co-issued), "
+                                      "profiles/stage_kernel_counters.json",
                 "kernel": f"ausm::stage_kernel (fused RK stage, {math_mode} math)",
                 "launch_us": launch_s * 1e6,
                 "launch_source": "HIP events over the timed region / (3 x steps)"}
