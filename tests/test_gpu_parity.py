@@ -74,8 +74,9 @@ def flow(mach, aoa, cfl=0.5, maxdt=1.0, K_p=0.0, K_u=0.75):
     return fs, par
 
 
-def new_pair(vtx, mach, aoa, scheme, limiter, math_mode="exact", refresh=1, tile=0, cfl=0.5):
-    fs, par = flow(mach, aoa, cfl=cfl)
+def new_pair(vtx, mach, aoa, scheme, limiter, math_mode="exact", refresh=1, tile=0, cfl=0.5,
+             K_p=0.0):
+    fs, par = flow(mach, aoa, cfl=cfl, K_p=K_p)
     N1, M, _ = vtx.shape
     s = ausm.Solver(M, N1 - 1)
     s.set_vertices(vtx)
@@ -133,6 +134,35 @@ def test_steps_exact_bitwise(scheme, mach, aoa):
         got = s.state()
         assert np.array_equal(got[1:N + 1], ref.state[1:N + 1]), (chunk, max_rel(got, ref.state))
         assert info["dt"] == ref.last_dt
+    s.close()
+
+
+@pytest.mark.parametrize("K_p,mach,aoa", [(0.25, 0.3, 2.0), (0.25, 0.8, 1.25), (1.0, 0.8, 1.25)])
+def test_ausmup_pressure_diffusion_bitwise(K_p, mach, aoa):
+    """AUSM+-up with its K_p pressure-diffusion term switched on (the
+    reference's default K_p is 0; the term is ausm.js:313): residual and 20 steps
+    bit-for-bit the oracle in exact math, the residual within the field
+    tolerance in fast math.  (K_p = 1 at Mach 0.3 is left out: with
+    K_p / f_a near 6 the explicit scheme turns non-finite within 10 steps,
+    in the oracle as on the GPU.)"""
+    M, N = 96, 40
+    vtx, _, _ = make_grid(M, N, code=2412, iters=150)
+    s, ref, par = new_pair(vtx, mach, aoa, AUSMUP, VANALBADA, K_p=K_p)
+    assert float(par[11]) == np.float32(K_p)
+    st = perturbed_state(par, M, N, seed=23)
+    s.set_state(st)
+    r_ref = Oracle(32).residual(par, AUSMUP, VANALBADA, vtx, st)
+    r_gpu = s.residual()
+    assert np.array_equal(r_gpu, r_ref), max_rel(r_gpu, r_ref)
+    s.set_math("fast")
+    assert max_rel(s.residual(), r_ref) <= FAST_TOL_FIELD
+    s.set_math("exact")
+    ref.set_state(st)
+    info = s.step(20)
+    ref.step(20)
+    got = s.state()
+    assert np.array_equal(got[1:N + 1], ref.state[1:N + 1]), max_rel(got, ref.state)
+    assert info["dt"] == ref.last_dt
     s.close()
 
 
