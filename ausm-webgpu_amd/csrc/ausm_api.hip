@@ -483,8 +483,16 @@ int ausm_grid_from_boundary(ausm_ctx* c, const double* body, const double* outer
   CHECK_HIP(hipMemcpyAsync(gy, hy.data(), bytes, hipMemcpyHostToDevice, c->stream), "grid upload");
   CHECK_HIP(launch_grid_interp(gx, gy, M, N, P, c->stream), "grid interpolation");
   if (method == AUSM_GRID_JACOBI) {
-    for (int it = 0; it < iters; ++it) {
-      CHECK_HIP(launch_grid_jacobi(gx, gy, hx2, hy2, M, N, P, c->stream), "grid Jacobi");
+    // whole blocks of sweeps in one launch each, the rest one per launch
+    const int B = grid_block_sweeps();
+    for (int it = 0; it < iters;) {
+      if (iters - it >= B) {
+        CHECK_HIP(launch_grid_jacobi_block(gx, gy, hx2, hy2, M, N, P, c->stream), "grid Jacobi");
+        it += B;
+      } else {
+        CHECK_HIP(launch_grid_jacobi(gx, gy, hx2, hy2, M, N, P, c->stream), "grid Jacobi");
+        ++it;
+      }
       std::swap(gx, hx2);
       std::swap(gy, hy2);
     }

@@ -28,18 +28,12 @@ __global__ void grid_interp_kernel(double* gx, double* gy, int M, int N, int P) 
   gy[y * P + x] = oy * (1.0 - t) + by * t;
 }
 
-// One Winslow/TTM point update with P = Q = 0 (shaders/grid.js:79-108).
-// Returns false when the local Jacobian is not negative (point kept).
-__device__ __forceinline__ bool ttm_point(const double* gx, const double* gy, int M, int P, int x,
-                                          int y, double& nx, double& ny) {
-  const int l = (x == 0) ? M - 1 : x - 1;
-  const int r = (x + 1 == M) ? 0 : x + 1;
-  const int u = (y + 1) * P, c = y * P, d = (y - 1) * P;
-  const double X[9] = {gx[c + x], gx[c + l], gx[c + r], gx[d + x], gx[u + x],
-                       gx[d + l], gx[d + r], gx[u + l], gx[u + r]};
-  const double Y[9] = {gy[c + x], gy[c + l], gy[c + r], gy[d + x], gy[u + x],
-                       gy[d + l], gy[d + r], gy[u + l], gy[u + r]};
-  // 0 center, 1 left, 2 right, 3 down, 4 up, 5 downleft, 6 downright, 7 upleft, 8 upright
+// One Winslow/TTM point update with P = Q = 0 (shaders/grid.js:79-108) on
+// the point's 3 x 3 stencil X, Y (0 centre, 1 left, 2 right, 3 down, 4 up,
+// 5 down-left, 6 down-right, 7 up-left, 8 up-right).  Returns false when the
+// local Jacobian is not negative (point kept).
+__device__ __forceinline__ bool ttm_update(const double (&X)[9], const double (&Y)[9], double& nx,
+                                           double& ny) {
   const double xix = 0.5 * (X[2] - X[1]), xiy = 0.5 * (Y[2] - Y[1]);
   const double etx = 0.5 * (X[4] - X[3]), ety = 0.5 * (Y[4] - Y[3]);
   const double xex = 0.25 * (((X[8] + X[5]) - X[7]) - X[6]);
@@ -66,6 +60,19 @@ __device__ __forceinline__ bool ttm_point(const double* gx, const double* gy, in
   return false;
 }
 
+// the same on the point's stencil in the vertex planes (i wraps at the seam)
+__device__ __forceinline__ bool ttm_point(const double* gx, const double* gy, int M, int P, int x,
+                                          int y, double& nx, double& ny) {
+  const int l = (x == 0) ? M - 1 : x - 1;
+  const int r = (x + 1 == M) ? 0 : x + 1;
+  const int u = (y + 1) * P, c = y * P, d = (y - 1) * P;
+  const double X[9] = {gx[c + x], gx[c + l], gx[c + r], gx[d + x], gx[u + x],
+                       gx[d + l], gx[d + r], gx[u + l], gx[u + r]};
+  const double Y[9] = {gy[c + x], gy[c + l], gy[c + r], gy[d + x], gy[u + x],
+                       gy[d + l], gy[d + r], gy[u + l], gy[u + r]};
+  return ttm_update(X, Y, nx, ny);
+}
+
 __global__ void grid_jacobi_kernel(const double* gx, const double* gy, double* ox, double* oy, int M,
                                    int N, int P) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,6 +87,81 @@ __global__ void grid_jacobi_kernel(const double* gx, const double* gy, double* o
   ttm_point(gx, gy, M, P, x, y, nx, ny);
   ox[y * P + x] = nx;
   oy[y * P + x] = ny;
+}
+
+// kGridSweeps Jacobi sweeps in one launch.  A workgroup loads its tile of
+// kGridTX x kGridTY vertices with a halo of kGridSweeps vertices on every
+// side (columns wrap around the seam; rows stop at the body and far-field
+// rows, which never move) into LDS and sweeps it there: sweep s updates the
+// vertices whose whole stencil was valid after sweep s - 1, a region that
+// shrinks by one vertex per sweep on every side except against a fixed
+// row, so that after kGridSweeps sweeps the tile itself is exact.  Every
+// vertex update sees the values a sweep-by-sweep run gives it, through the
+// same arithmetic (ttm_update): the result is bit-for-bit that of
+// kGridSweeps launches of grid_jacobi_kernel.
+constexpr int kGridSweeps = 4, kGridTX = 64, kGridTY = 8;
+constexpr int kGridLX = kGridTX + 2 * kGridSweeps, kGridLY = kGridTY + 2 * kGridSweeps;
+
+__global__ __launch_bounds__(256) void grid_jacobi_block_kernel(const double* gx, const double* gy,
+                                                                double* ox, double* oy, int M, int N,
+                                                                int P, int tiles_x) {
+  __shared__ double sx[2][kGridLY][kGridLX], sy[2][kGridLY][kGridLX];
+  const int x0 = (blockIdx.x % tiles_x) * kGridTX;
+  const int y0 = (blockIdx.x / tiles_x) * kGridTY;
+  const int ga = max(y0 - kGridSweeps, 0);                // first row in LDS
+  const int gb = min(y0 + kGridTY + kGridSweeps, N + 1);  // one past the last
+  for (int t = threadIdx.x; t < (gb - ga) * kGridLX; t += 256) {
+    const int r = t / kGridLX, c = t - r * kGridLX;
+    int x = (x0 - kGridSweeps + c) % M;
+    if (x < 0) x += M;
+    const size_t g = (size_t)(ga + r) * (size_t)P + (size_t)x;
+    sx[0][r][c] = gx[g];
+    sy[0][r][c] = gy[g];
+  }
+  __syncthreads();
+  // rows [va, vb) (grid numbering) hold valid values after each sweep
+  int va = ga, vb = gb;
+#pragma unroll
+  for (int s = 1; s <= kGridSweeps; ++s) {
+    const int cur = (s - 1) & 1, nxt = s & 1;
+    const int na = ga == 0 ? 0 : va + 1;
+    const int nb = gb == N + 1 ? N + 1 : vb - 1;
+    const int w = kGridLX - 2 * s;  // columns [s, kGridLX - s)
+    for (int t = threadIdx.x; t < (nb - na) * w; t += 256) {
+      const int rr = t / w;
+      const int c = s + (t - rr * w);
+      const int gr = na + rr, r = gr - ga;
+      double nx, ny;
+      if (gr == 0 || gr == N) {
+        nx = sx[cur][r][c];
+        ny = sy[cur][r][c];
+      } else {
+        const double X[9] = {sx[cur][r][c],         sx[cur][r][c - 1],     sx[cur][r][c + 1],
+                             sx[cur][r - 1][c],     sx[cur][r + 1][c],     sx[cur][r - 1][c - 1],
+                             sx[cur][r - 1][c + 1], sx[cur][r + 1][c - 1], sx[cur][r + 1][c + 1]};
+        const double Y[9] = {sy[cur][r][c],         sy[cur][r][c - 1],     sy[cur][r][c + 1],
+                             sy[cur][r - 1][c],     sy[cur][r + 1][c],     sy[cur][r - 1][c - 1],
+                             sy[cur][r - 1][c + 1], sy[cur][r + 1][c - 1], sy[cur][r + 1][c + 1]};
+        ttm_update(X, Y, nx, ny);
+      }
+      sx[nxt][r][c] = nx;
+      sy[nxt][r][c] = ny;
+    }
+    va = na;
+    vb = nb;
+    __syncthreads();
+  }
+  constexpr int fin = kGridSweeps & 1;
+  const int rows_out = min(y0 + kGridTY, N + 1) - y0;
+  for (int t = threadIdx.x; t < rows_out * kGridTX; t += 256) {
+    const int rr = t / kGridTX, cc = t - rr * kGridTX;
+    const int x = x0 + cc;
+    if (x >= M) continue;
+    const int gr = y0 + rr;
+    const size_t g = (size_t)gr * (size_t)P + (size_t)x;
+    ox[g] = sx[fin][gr - ga][kGridSweeps + cc];
+    oy[g] = sy[fin][gr - ga][kGridSweeps + cc];
+  }
 }
 
 // Red-black successive over-relaxation of the same update (in place).
@@ -448,6 +530,15 @@ hipError_t launch_grid_jacobi(const double* gx, const double* gy, double* ox, do
                               int N, int P, hipStream_t s) {
   hipLaunchKernelGGL(grid_jacobi_kernel, rows_grid(M, N + 1), dim3(256), 0, s, gx, gy, ox, oy, M, N,
                      P);
+  return hipGetLastError();
+}
+int grid_block_sweeps() { return kGridSweeps; }
+hipError_t launch_grid_jacobi_block(const double* gx, const double* gy, double* ox, double* oy,
+                                    int M, int N, int P, hipStream_t s) {
+  const int tiles_x = (M + kGridTX - 1) / kGridTX;
+  const int tiles_y = (N + 1 + kGridTY - 1) / kGridTY;
+  hipLaunchKernelGGL(grid_jacobi_block_kernel, dim3(tiles_x * tiles_y), dim3(256), 0, s, gx, gy, ox,
+                     oy, M, N, P, tiles_x);
   return hipGetLastError();
 }
 hipError_t launch_grid_sor(double* gx, double* gy, int M, int N, int P, int color, double omega,

@@ -9,6 +9,11 @@ namespace ausm {
 hipError_t launch_grid_interp(double* gx, double* gy, int M, int N, int P, hipStream_t s);
 hipError_t launch_grid_jacobi(const double* gx, const double* gy, double* ox, double* oy, int M,
                               int N, int P, hipStream_t s);
+// grid_block_sweeps() Jacobi sweeps in one launch, bit-for-bit as many
+// launch_grid_jacobi calls
+int grid_block_sweeps();
+hipError_t launch_grid_jacobi_block(const double* gx, const double* gy, double* ox, double* oy,
+                                    int M, int N, int P, hipStream_t s);
 hipError_t launch_grid_sor(double* gx, double* gy, int M, int N, int P, int color, double omega,
                            hipStream_t s);
 hipError_t launch_f64_to_f32(const double* a, float* b, size_t n, hipStream_t s);

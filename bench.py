@@ -201,6 +201,22 @@ def cpu_baseline(vtx, M, N, target_s=15.0, q_gpu=None, r_gpu=None):
                               "sample": f"{n1} steps, {dt1:.1f} s"}}
 
 
+def grid_solve_record(M, N, sweeps, seconds):
+    """The FP64 elliptic grid solve that built the workload's grid, timed on
+    the host around the whole call (upload, interpolation, the Jacobi sweeps
+    and the FP32 vertex and metric planes): sweeps per second and the sweeps'
+    algorithmic traffic, 32 B per vertex per sweep (both FP64 planes read and
+    written)."""
+    verts = (N + 1) * M
+    gbps = 32.0 * verts * sweeps / seconds / 1e9 if seconds > 0 else None
+    return {"vertices": verts, "sweeps": sweeps, "seconds": seconds,
+            "us_per_sweep": seconds / max(sweeps, 1) * 1e6,
+            "achieved_GBps": gbps, "frac_hbm": None if gbps is None else gbps / HBM_PEAK_GBPS,
+            "kernel": "ausm::grid_jacobi_block_kernel (four FP64 Jacobi sweeps per launch "
+                      "through LDS)",
+            "source": "host clock around ausm_grid_from_boundary"}
+
+
 def host_cpu():
     """The host's CPU model and the logical CPUs this process may run on."""
     model = None
@@ -631,6 +647,7 @@ def main():
         "pct_hbm_roofline_whole_step": 100.0 * whole_gbps / HBM_PEAK_GBPS,
         "steps_per_s": args.steps / (ms / 1e3),
         "slau2_cell_steps_per_s": slau2[args.math],
+        "grid_solve": grid_solve_record(M, N, args.grid_iters, t_grid),
         "roofline": dict(roofline(args.math, avg_launch_s),
                          isolated_stage_us=[x * 1e3 for x in stage_ms]),
         "parity": parity,

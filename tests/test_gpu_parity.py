@@ -313,16 +313,21 @@ def test_fast_math_limiters(limiter):
     s.close()
 
 
-def test_grid_fp64_bitwise():
-    M, N = 80, 30
+@pytest.mark.parametrize("M,N,iters", [(80, 30, 301), (4, 4, 9), (64, 7, 13), (130, 17, 40),
+                                       (512, 384, 64)])
+def test_grid_fp64_bitwise(M, N, iters):
+    """The FP64 Jacobi grid solve (four sweeps per launch through LDS, the
+    rest one per launch) bit-for-bit the FP64 oracle's sweep-by-sweep solve,
+    on rings narrower and wider than one 64-column tile, ring heights within
+    and across 8-row tiles, and sweep counts with and without a remainder."""
     m, p, t = naca4_digits(2412)
     body = naca4_boundary(M, m, p, t, size=0.5, expo=1.4, spacing=0)
     outer = circle_boundary(M, 8.0)
     s = ausm.Solver(M, N)
     s.update_grid_boundaries(body, outer)
-    s.prepare_grid(iters=301)
+    s.prepare_grid(iters=iters)
     v64 = s.vertices(fp64=True)
-    ref = Oracle(64).grid(body, outer, N, 301)
+    ref = Oracle(64).grid(body, outer, N, iters)
     assert np.array_equal(v64, ref)
     assert np.array_equal(s.vertices(), ref.astype(np.float32))
     # metrics from the FP32 vertices, as the reference's finalize pass
