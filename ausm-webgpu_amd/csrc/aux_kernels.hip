@@ -99,7 +99,7 @@ __global__ void grid_jacobi_kernel(const double* gx, const double* gy, double* o
 // vertex update sees the values a sweep-by-sweep run gives it, through the
 // same arithmetic (ttm_update): the result is bit-for-bit that of
 // kGridSweeps launches of grid_jacobi_kernel.
-constexpr int kGridSweeps = 4, kGridTX = 64, kGridTY = 8;
+constexpr int kGridSweeps = 4, kGridTX = 32, kGridTY = 8;
 constexpr int kGridLX = kGridTX + 2 * kGridSweeps, kGridLY = kGridTY + 2 * kGridSweeps;
 
 __global__ __launch_bounds__(256) void grid_jacobi_block_kernel(const double* gx, const double* gy,
