@@ -552,6 +552,11 @@ def main():
     ms, wall, info = time_steps(s, ev, args.steps, args.warmup)
     # the other arithmetic beside it (same steps, same warm-up), straight after
     ms_o, _, _ = time_steps(so, ev, args.steps, args.warmup)
+    # supplementary: each stage alone, launched back to back on warm caches
+    # (and a warm GPU: straight after the timed runs)
+    ms3 = (ctypes.c_float * 3)()
+    ausm.lib.check(s._L.ausm_profile_stages(s.h, 200, ms3))
+    stage_ms = [ms3[0], ms3[1], ms3[2]]
     s2.close()
     so.close()
     rate = M * N * args.steps / (ms / 1e3)
@@ -574,10 +579,6 @@ def main():
     per_launch_bytes = ALG_BYTES_PER_CELL_STEP / 3 * M * N
     s.set_math(args.math)
 
-    # supplementary: each stage alone, launched back to back on warm caches
-    ms3 = (ctypes.c_float * 3)()
-    ausm.lib.check(s._L.ausm_profile_stages(s.h, 200, ms3))
-    stage_ms = [ms3[0], ms3[1], ms3[2]]
     log(f"[bench] stage kernel {avg_launch_s * 1e6:.2f} us per launch in the timed region; "
         f"isolated stages {[round(x * 1e3, 2) for x in stage_ms]} us")
 
