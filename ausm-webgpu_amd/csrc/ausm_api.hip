@@ -76,7 +76,7 @@ void device_arithmetic_store(int device, long long bad) {
   g_arith_run[device].store(true);
 }
 
-int device_arithmetic_verdict(int device, long long bad) {
+int device_arithmetic_status(int device, long long bad) {
   if (bad == 0) return AUSM_OK;
   return set_error(AUSM_ERR_HIP,
                    "device %d: the short division or square root of the exact arithmetic differs "
@@ -90,7 +90,7 @@ int device_arithmetic_ok(int device) {
     if (r != AUSM_OK) return r;
     device_arithmetic_store(device, bad);
   }
-  return device_arithmetic_verdict(device, bad);
+  return device_arithmetic_status(device, bad);
 }
 
 // Host-side parameter block -> device copy (the Uniforms upload,

@@ -73,7 +73,7 @@ int device_arithmetic_ok(int device);
 // has run; storing a count; the error (or AUSM_OK) for a count
 long long device_arithmetic_cached(int device);
 void device_arithmetic_store(int device, long long bad);
-int device_arithmetic_verdict(int device, long long bad);
+int device_arithmetic_status(int device, long long bad);
 size_t round_up(size_t x, size_t a);
 // the FP32 parameter block of a free stream (inState formed in double,
 // stored as FP32, as the reference's host does)

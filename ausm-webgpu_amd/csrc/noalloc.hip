@@ -194,7 +194,7 @@ extern "C" int ausm_step_noalloc(int device, const ausm_step_desc* d, const floa
       bad = (long long)h;
       device_arithmetic_store(device, bad);
     }
-    const int v = device_arithmetic_verdict(device, bad);
+    const int v = device_arithmetic_status(device, bad);
     if (v != AUSM_OK) return v;
   }
 
