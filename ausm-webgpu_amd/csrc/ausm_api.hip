@@ -498,8 +498,11 @@ int ausm_grid_from_boundary(ausm_ctx* c, const double* body, const double* outer
     }
   } else {
     for (int it = 0; it < iters; ++it) {
-      CHECK_HIP(launch_grid_sor(gx, gy, M, N, P, 0, omega, c->stream), "grid SOR");
-      CHECK_HIP(launch_grid_sor(gx, gy, M, N, P, 1, omega, c->stream), "grid SOR");
+      for (int color = 0; color < 2; ++color) {
+        CHECK_HIP(launch_grid_sor(gx, gy, hx2, hy2, M, N, P, color, omega, c->stream), "grid SOR");
+        std::swap(gx, hx2);
+        std::swap(gy, hy2);
+      }
     }
   }
   c->g_cur = (gx == c->g64[0]) ? 0 : 2;

@@ -164,18 +164,28 @@ __global__ __launch_bounds__(256) void grid_jacobi_block_kernel(const double* gx
   }
 }
 
-// Red-black successive over-relaxation of the same update (in place).
-__global__ void grid_sor_kernel(double* gx, double* gy, int M, int N, int P, int color,
-                                double omega) {
+// Red-black over-relaxation of the same update.  A half-sweep relaxes the
+// interior vertices of one colour ((x + y) & 1) from the values after the
+// previous half-sweep (gx, gy) and copies every other vertex, writing
+// ox, oy: the stencil's diagonal neighbours have the vertex's own colour, so
+// relaxing in place would let a vertex read neighbours that the same
+// half-sweep may or may not have updated yet.
+__global__ void grid_sor_kernel(const double* gx, const double* gy, double* ox, double* oy, int M,
+                                int N, int P, int color, double omega) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y + 1;
-  if (x >= M || y >= N || ((x + y) & 1) != color) return;
-  double nx, ny;
-  if (ttm_point(gx, gy, M, P, x, y, nx, ny)) {
-    const double cx = gx[y * P + x], cy = gy[y * P + x];
-    gx[y * P + x] = cx + omega * (nx - cx);
-    gy[y * P + x] = cy + omega * (ny - cy);
+  const int y = blockIdx.y;
+  if (x >= M || y > N) return;
+  const size_t c = (size_t)y * (size_t)P + (size_t)x;
+  double vx = gx[c], vy = gy[c];
+  if (y > 0 && y < N && ((x + y) & 1) == color) {
+    double nx, ny;
+    if (ttm_point(gx, gy, M, P, x, y, nx, ny)) {
+      vx = vx + omega * (nx - vx);
+      vy = vy + omega * (ny - vy);
+    }
   }
+  ox[c] = vx;
+  oy[c] = vy;
 }
 
 __global__ void f64_to_f32_kernel(const double* a, float* b, size_t n) {
@@ -541,11 +551,10 @@ hipError_t launch_grid_jacobi_block(const double* gx, const double* gy, double* 
                      oy, M, N, P, tiles_x);
   return hipGetLastError();
 }
-hipError_t launch_grid_sor(double* gx, double* gy, int M, int N, int P, int color, double omega,
-                           hipStream_t s) {
-  if (N < 2) return hipSuccess;
-  hipLaunchKernelGGL(grid_sor_kernel, rows_grid(M, N - 1), dim3(256), 0, s, gx, gy, M, N, P, color,
-                     omega);
+hipError_t launch_grid_sor(const double* gx, const double* gy, double* ox, double* oy, int M,
+                           int N, int P, int color, double omega, hipStream_t s) {
+  hipLaunchKernelGGL(grid_sor_kernel, rows_grid(M, N + 1), dim3(256), 0, s, gx, gy, ox, oy, M, N, P,
+                     color, omega);
   return hipGetLastError();
 }
 hipError_t launch_f64_to_f32(const double* a, float* b, size_t n, hipStream_t s) {

@@ -14,8 +14,9 @@ hipError_t launch_grid_jacobi(const double* gx, const double* gy, double* ox, do
 int grid_block_sweeps();
 hipError_t launch_grid_jacobi_block(const double* gx, const double* gy, double* ox, double* oy,
                                     int M, int N, int P, hipStream_t s);
-hipError_t launch_grid_sor(double* gx, double* gy, int M, int N, int P, int color, double omega,
-                           hipStream_t s);
+// one red-black half-sweep of colour `color`, from (gx, gy) into (ox, oy)
+hipError_t launch_grid_sor(const double* gx, const double* gy, double* ox, double* oy, int M,
+                           int N, int P, int color, double omega, hipStream_t s);
 hipError_t launch_f64_to_f32(const double* a, float* b, size_t n, hipStream_t s);
 hipError_t launch_metrics(const float* vx, const float* vy, int M, int N, int P, float* area,
                           float* flen, hipStream_t s);

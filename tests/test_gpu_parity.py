@@ -352,6 +352,32 @@ def test_grid_sor_converges_to_jacobi_fixed_point():
     s.close()
 
 
+def test_grid_sor_is_deterministic():
+    """Each red-black half-sweep reads the previous half-sweep's buffer and
+    writes the other one, so the result does not depend on the order in
+    which the GPU updates the vertices: two solves of the same body give the
+    same FP64 grid bit for bit, and it approaches the Jacobi fixed point."""
+    M, N = 96, 40
+    m, p, t = naca4_digits(2412)
+    body = naca4_boundary(M, m, p, t, size=0.5, expo=1.4, spacing=0)
+    outer = circle_boundary(M, 8.0)
+    grids = []
+    for _ in range(2):
+        s = ausm.Solver(M, N)
+        s.update_grid_boundaries(body, outer)
+        s.prepare_grid(iters=2000, method="sor", omega=1.5)
+        grids.append(s.vertices(fp64=True))
+        s.close()
+    assert np.array_equal(grids[0], grids[1])
+    # toward the fixed point the Jacobi sweeps approach (20000 sweeps), and
+    # faster than 2000 of them (which are still 4 % away)
+    ref = Oracle(64).grid(body, outer, N, 20000)
+    jac = Oracle(64).grid(body, outer, N, 2000)
+    err_sor = np.abs(grids[0] - ref).max()
+    err_jac = np.abs(jac - ref).max()
+    assert err_sor < 0.05 * err_jac, (err_sor, err_jac)
+
+
 def test_wave_speeds_forces_bitwise():
     M, N = 96, 40
     vtx, _, _ = make_grid(M, N, code=2412, iters=150)
