@@ -70,7 +70,9 @@ enum ausm_init { AUSM_INIT_REST = 0, AUSM_INIT_FREESTREAM = 1 };
 
 /* elliptic grid smoother: JACOBI is the reference's ping-pong sweep
  * (src/shaders/grid.js:44-110); SOR is red-black over-relaxation of the same
- * update (needs even M). */
+ * update, each half-sweep relaxing one colour from the previous half-sweep's
+ * values (deterministic; needs even M for the colouring to hold across the
+ * seam). */
 enum ausm_grid_method { AUSM_GRID_JACOBI = 0, AUSM_GRID_SOR = 1 };
 
 /* body-point spacing along the chord */
