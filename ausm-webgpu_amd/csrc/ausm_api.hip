@@ -497,12 +497,19 @@ int ausm_grid_from_boundary(ausm_ctx* c, const double* body, const double* outer
       std::swap(gy, hy2);
     }
   } else {
-    for (int it = 0; it < iters; ++it) {
-      for (int color = 0; color < 2; ++color) {
-        CHECK_HIP(launch_grid_sor(gx, gy, hx2, hy2, M, N, P, color, omega, c->stream), "grid SOR");
-        std::swap(gx, hx2);
-        std::swap(gy, hy2);
+    // half-sweeps h = 0 .. 2 iters - 1 of colour h & 1, in blocks as above
+    const int B = grid_block_sweeps();
+    for (int h = 0; h < 2 * iters;) {
+      if (2 * iters - h >= B) {
+        CHECK_HIP(launch_grid_sor_block(gx, gy, hx2, hy2, M, N, P, h & 1, omega, c->stream),
+                  "grid SOR");
+        h += B;
+      } else {
+        CHECK_HIP(launch_grid_sor(gx, gy, hx2, hy2, M, N, P, h & 1, omega, c->stream), "grid SOR");
+        ++h;
       }
+      std::swap(gx, hx2);
+      std::swap(gy, hy2);
     }
   }
   c->g_cur = (gx == c->g64[0]) ? 0 : 2;

@@ -89,7 +89,8 @@ __global__ void grid_jacobi_kernel(const double* gx, const double* gy, double* o
   oy[y * P + x] = ny;
 }
 
-// kGridSweeps Jacobi sweeps in one launch.  A workgroup loads its tile of
+// kGridSweeps sweeps in one launch: Jacobi sweeps, or (SOR) red-black
+// half-sweeps starting with colour color0.  A workgroup loads its tile of
 // kGridTX x kGridTY vertices with a halo of kGridSweeps vertices on every
 // side (columns wrap around the seam; rows stop at the body and far-field
 // rows, which never move) into LDS and sweeps it there: sweep s updates the
@@ -97,14 +98,15 @@ __global__ void grid_jacobi_kernel(const double* gx, const double* gy, double* o
 // shrinks by one vertex per sweep on every side except against a fixed
 // row, so that after kGridSweeps sweeps the tile itself is exact.  Every
 // vertex update sees the values a sweep-by-sweep run gives it, through the
-// same arithmetic (ttm_update): the result is bit-for-bit that of
-// kGridSweeps launches of grid_jacobi_kernel.
+// same arithmetic: the result is bit-for-bit that of kGridSweeps launches of
+// grid_jacobi_kernel, or of grid_sor_kernel with alternating colours.
 constexpr int kGridSweeps = 4, kGridTX = 32, kGridTY = 8;
 constexpr int kGridLX = kGridTX + 2 * kGridSweeps, kGridLY = kGridTY + 2 * kGridSweeps;
 
-__global__ __launch_bounds__(256) void grid_jacobi_block_kernel(const double* gx, const double* gy,
-                                                                double* ox, double* oy, int M, int N,
-                                                                int P, int tiles_x) {
+template <bool SOR>
+__global__ __launch_bounds__(256) void grid_block_kernel(const double* gx, const double* gy,
+                                                         double* ox, double* oy, int M, int N, int P,
+                                                         int tiles_x, int color0, double omega) {
   __shared__ double sx[2][kGridLY][kGridLX], sy[2][kGridLY][kGridLX];
   const int x0 = (blockIdx.x % tiles_x) * kGridTX;
   const int y0 = (blockIdx.x / tiles_x) * kGridTY;
@@ -131,18 +133,30 @@ __global__ __launch_bounds__(256) void grid_jacobi_block_kernel(const double* gx
       const int rr = t / w;
       const int c = s + (t - rr * w);
       const int gr = na + rr, r = gr - ga;
-      double nx, ny;
-      if (gr == 0 || gr == N) {
-        nx = sx[cur][r][c];
-        ny = sy[cur][r][c];
-      } else {
+      bool relax = gr != 0 && gr != N;
+      if constexpr (SOR) {
+        // this half-sweep's colour, on the vertex's own column of the ring
+        int x = (x0 - kGridSweeps + c) % M;
+        if (x < 0) x += M;
+        relax = relax && ((x + gr) & 1) == ((color0 + s - 1) & 1);
+      }
+      double nx = sx[cur][r][c], ny = sy[cur][r][c];
+      if (relax) {
         const double X[9] = {sx[cur][r][c],         sx[cur][r][c - 1],     sx[cur][r][c + 1],
                              sx[cur][r - 1][c],     sx[cur][r + 1][c],     sx[cur][r - 1][c - 1],
                              sx[cur][r - 1][c + 1], sx[cur][r + 1][c - 1], sx[cur][r + 1][c + 1]};
         const double Y[9] = {sy[cur][r][c],         sy[cur][r][c - 1],     sy[cur][r][c + 1],
                              sy[cur][r - 1][c],     sy[cur][r + 1][c],     sy[cur][r - 1][c - 1],
                              sy[cur][r - 1][c + 1], sy[cur][r + 1][c - 1], sy[cur][r + 1][c + 1]};
-        ttm_update(X, Y, nx, ny);
+        if constexpr (SOR) {
+          double tx, ty;
+          if (ttm_update(X, Y, tx, ty)) {
+            nx = nx + omega * (tx - nx);
+            ny = ny + omega * (ty - ny);
+          }
+        } else {
+          ttm_update(X, Y, nx, ny);
+        }
       }
       sx[nxt][r][c] = nx;
       sy[nxt][r][c] = ny;
@@ -547,8 +561,16 @@ hipError_t launch_grid_jacobi_block(const double* gx, const double* gy, double* 
                                     int M, int N, int P, hipStream_t s) {
   const int tiles_x = (M + kGridTX - 1) / kGridTX;
   const int tiles_y = (N + 1 + kGridTY - 1) / kGridTY;
-  hipLaunchKernelGGL(grid_jacobi_block_kernel, dim3(tiles_x * tiles_y), dim3(256), 0, s, gx, gy, ox,
-                     oy, M, N, P, tiles_x);
+  hipLaunchKernelGGL(grid_block_kernel<false>, dim3(tiles_x * tiles_y), dim3(256), 0, s, gx, gy, ox,
+                     oy, M, N, P, tiles_x, 0, 1.0);
+  return hipGetLastError();
+}
+hipError_t launch_grid_sor_block(const double* gx, const double* gy, double* ox, double* oy, int M,
+                                 int N, int P, int color0, double omega, hipStream_t s) {
+  const int tiles_x = (M + kGridTX - 1) / kGridTX;
+  const int tiles_y = (N + 1 + kGridTY - 1) / kGridTY;
+  hipLaunchKernelGGL(grid_block_kernel<true>, dim3(tiles_x * tiles_y), dim3(256), 0, s, gx, gy, ox,
+                     oy, M, N, P, tiles_x, color0, omega);
   return hipGetLastError();
 }
 hipError_t launch_grid_sor(const double* gx, const double* gy, double* ox, double* oy, int M,

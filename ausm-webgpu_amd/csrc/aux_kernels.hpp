@@ -9,11 +9,14 @@ namespace ausm {
 hipError_t launch_grid_interp(double* gx, double* gy, int M, int N, int P, hipStream_t s);
 hipError_t launch_grid_jacobi(const double* gx, const double* gy, double* ox, double* oy, int M,
                               int N, int P, hipStream_t s);
-// grid_block_sweeps() Jacobi sweeps in one launch, bit-for-bit as many
-// launch_grid_jacobi calls
+// grid_block_sweeps() Jacobi sweeps (or red-black half-sweeps, colours
+// alternating from color0) in one launch, bit-for-bit as many
+// launch_grid_jacobi (launch_grid_sor) calls
 int grid_block_sweeps();
 hipError_t launch_grid_jacobi_block(const double* gx, const double* gy, double* ox, double* oy,
                                     int M, int N, int P, hipStream_t s);
+hipError_t launch_grid_sor_block(const double* gx, const double* gy, double* ox, double* oy, int M,
+                                 int N, int P, int color0, double omega, hipStream_t s);
 // one red-black half-sweep of colour `color`, from (gx, gy) into (ox, oy)
 hipError_t launch_grid_sor(const double* gx, const double* gy, double* ox, double* oy, int M,
                            int N, int P, int color, double omega, hipStream_t s);

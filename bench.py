@@ -212,7 +212,7 @@ def grid_solve_record(M, N, sweeps, seconds):
     return {"vertices": verts, "sweeps": sweeps, "seconds": seconds,
             "us_per_sweep": seconds / max(sweeps, 1) * 1e6,
             "achieved_GBps": gbps, "frac_hbm": None if gbps is None else gbps / HBM_PEAK_GBPS,
-            "kernel": "ausm::grid_jacobi_block_kernel (four FP64 Jacobi sweeps per launch "
+            "kernel": "ausm::grid_block_kernel<false> (four FP64 Jacobi sweeps per launch "
                       "through LDS)",
             "source": "host clock around ausm_grid_from_boundary"}
 

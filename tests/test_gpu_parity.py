@@ -378,6 +378,58 @@ def test_grid_sor_is_deterministic():
     assert err_sor < 0.05 * err_jac, (err_sor, err_jac)
 
 
+def _sor_half_sweep(gx, gy, color, omega):
+    """One red-black half-sweep on (N+1) x M FP64 vertex planes in numpy, in
+    the kernel's operation order: the colour's interior vertices relaxed from
+    the input planes (i periodic), the others copied."""
+    def nb(a):
+        l, r = np.roll(a, 1, axis=1), np.roll(a, -1, axis=1)
+        return [a[1:-1], l[1:-1], r[1:-1], a[:-2], a[2:], l[:-2], r[:-2], l[2:], r[2:]]
+    X, Y = nb(gx), nb(gy)
+    xix, xiy = 0.5 * (X[2] - X[1]), 0.5 * (Y[2] - Y[1])
+    etx, ety = 0.5 * (X[4] - X[3]), 0.5 * (Y[4] - Y[3])
+    xex = 0.25 * (((X[8] + X[5]) - X[7]) - X[6])
+    xey = 0.25 * (((Y[8] + Y[5]) - Y[7]) - Y[6])
+    alpha, gam = etx * etx + ety * ety, xix * xix + xiy * xiy
+    beta, jac = xix * etx + xiy * ety, xix * ety - xiy * etx
+    den = 2.0 * (alpha + gam)
+    npx = (alpha * ((X[1] + X[2]) + 0.0 * xix) + gam * ((X[3] + X[4]) + 0.0 * etx)
+           - (2.0 * beta) * xex) / den
+    npy = (alpha * ((Y[1] + Y[2]) + 0.0 * xiy) + gam * ((Y[3] + Y[4]) + 0.0 * ety)
+           - (2.0 * beta) * xey) / den
+    nx = X[0] * (1.0 - 1.0) + npx * 1.0
+    ny = Y[0] * (1.0 - 1.0) + npy * 1.0
+    N1, M = gx.shape
+    yy, xx = np.meshgrid(np.arange(1, N1 - 1), np.arange(M), indexing="ij")
+    on = (((xx + yy) & 1) == color) & (jac < 0.0)
+    ox, oy = gx.copy(), gy.copy()
+    ox[1:-1] = np.where(on, X[0] + omega * (nx - X[0]), X[0])
+    oy[1:-1] = np.where(on, Y[0] + omega * (ny - Y[0]), Y[0])
+    return ox, oy
+
+
+@pytest.mark.parametrize("M,N,iters", [(64, 7, 3), (96, 40, 5), (70, 17, 2)])
+def test_grid_sor_blocks_match_half_sweeps(M, N, iters):
+    """SOR runs its half-sweeps four per launch through LDS (the rest one per
+    launch): bit-for-bit the half-sweeps one after another, evaluated here
+    in numpy in the kernel's operation order."""
+    m, p, t = naca4_digits(2412)
+    body = naca4_boundary(M, m, p, t, size=0.5, expo=1.4, spacing=0)
+    outer = circle_boundary(M, 8.0)
+    s = ausm.Solver(M, N)
+    s.update_grid_boundaries(body, outer)
+    s.prepare_grid(iters=0)
+    v0 = s.vertices(fp64=True)
+    s.prepare_grid(iters=iters, method="sor", omega=1.5)
+    v = s.vertices(fp64=True)
+    s.close()
+    gx, gy = v0[..., 0].copy(), v0[..., 1].copy()
+    for h in range(2 * iters):
+        gx, gy = _sor_half_sweep(gx, gy, h & 1, 1.5)
+    assert not np.array_equal(gx, v0[..., 0])
+    assert np.array_equal(v[..., 0], gx) and np.array_equal(v[..., 1], gy)
+
+
 def test_wave_speeds_forces_bitwise():
     M, N = 96, 40
     vtx, _, _ = make_grid(M, N, code=2412, iters=150)
