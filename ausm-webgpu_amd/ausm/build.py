@@ -38,7 +38,9 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=
 
 # per-file additions: the stage kernels (single-grid and batched) are
 # scheduled for instruction-level parallelism (single-grid 1-3 % faster
-# across sizes, 7 % for exact math; the 64-solve batched mix 1.5 %)
+# across sizes, 7 % for exact math; the 64-solve batched mix 1.5 %; the
+# max-memory-clause strategy ran the exact 512 x 384 step 4 % slower, and
+# the iterative-ilp strategy does not compile these kernels)
 EXTRA_FLAGS = {src: ["-mllvm", "-amdgpu-sched-strategy=max-ilp"] for src in STAGE_SOURCES + BATCH_SOURCES}
 
 
